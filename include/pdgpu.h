@@ -1,0 +1,215 @@
+/*
+ * pdgpu.h -- C ABI of the MI355X (gfx950) nonbonded / GB-SA force engine for PD.
+ *
+ * One engine handle holds one molecular system resident in HBM together with
+ * its neighbour list, and plays the parts of three PD objects:
+ *   - the neighbour list, a WorkSpace component (neighbourlist.h:54-126,
+ *     NeighbourList::calcNewList neighbourlist.cpp:222-244, with
+ *     WorkSpace::cleanSpace workspace.cpp:371-380 run before each rebuild as
+ *     ProtocolBase::refreshNeighborList does, protocolbase.cpp:135-144);
+ *   - the forcefield components FF_NonBonded (nonbonded.h:62-231),
+ *     FF_GeneralizedBorn_Still (gbff.h:123-203) and FF_Bonded (ffbonded.h), behind
+ *     ForcefieldBase::calcForces / calcEnergies (forcefield.h:110-239);
+ *   - the MolecularDynamics integrators (md.cpp:805-855 velocity Verlet,
+ *     md.cpp:923-1059 Langevin) run as a device-resident loop.
+ *
+ * Conventions (all entry points):
+ *   - PD units at the boundary: positions in Angstrom, forces in J/m, energies in
+ *     J per molecule, masses in kg, velocities in m/s, per-atom LJ epsilon as
+ *     sqrt(epsilon[J]) (ffparam.cpp:1216), radius = R_min/2 in Angstrom.
+ *   - Coordinate arrays are AoS xyz doubles with a caller-given byte stride, so
+ *     PD's SnapShotAtom array (p, f, v dvectors, stride 72 bytes, snapshot.h:33-39)
+ *     can be passed without copying into a temporary.
+ *   - Every call returns PDGPU_OK (0) or a negative error code; the text of the
+ *     last error of a handle is pdgpu_last_error(handle).  No C++ exception
+ *     crosses this interface.  One handle is used by one host thread at a time.
+ */
+#ifndef PDGPU_H
+#define PDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDGPU_OK 0
+#define PDGPU_ERR_ARG (-1)      /* invalid argument or call order */
+#define PDGPU_ERR_HIP (-2)      /* HIP runtime error */
+#define PDGPU_ERR_SETUP (-3)    /* configuration PD itself rejects (e.g. >32 image vectors) */
+#define PDGPU_ERR_NOMEM (-4)    /* device allocation failed */
+#define PDGPU_ERR_UNSTABLE (-5) /* NaN energy: ProtocolBase::assertStability (protocolbase.cpp:126-132) */
+
+typedef struct pdgpu_engine pdgpu_engine;
+
+/* Energies of one evaluation; members mirror PD's Hamiltonian (hamiltonian.h:40-66). */
+typedef struct pdgpu_energies {
+  double epot;
+  double epot_vdw;
+  double epot_elec;
+  double epot_pol;
+  double epot_pol_self;
+  double epot_pol_cross;
+  double epot_bond;
+  double epot_angle;
+  double epot_torsion;
+  double epot_surf;
+  double virial; /* InternalVirial, sum of r * dE/dr [J] */
+  double ekin;
+} pdgpu_energies;
+
+/* FF_NonBonded public parameters (nonbonded.h:72-143; defaults nonbonded.cpp:985-1021). */
+typedef struct pdgpu_nb_params {
+  double Cutoff, InnerCutoff, VdwCutoff, VdwInnerCutoff;
+  double Vdw14Scaling, Elec14Scaling;
+  double Dielectric, DDDielectricAlpha;
+  int32_t DoVdw, DoElec, ForceSwitch, EnergySwitch;
+  int32_t VdwCor;
+  double VdwCorDensity, VdwCorEpsilon, VdwCorRadius, VdwCorCutoff;
+} pdgpu_nb_params;
+
+/* FF_GeneralizedBorn_Still parameters (gbff.cpp:34-55) and its inherited
+ * nonbonded cutoffs and 1-4 scalings. */
+typedef struct pdgpu_gb_params {
+  double Cutoff, InnerCutoff, VdwCutoff, VdwInnerCutoff;
+  double Vdw14Scaling, Elec14Scaling;
+  double DielectricSolute, DielectricSolvent;
+  double GbsaStillCutoff, ExpApproxThreshold, DielectricOffset, BornRadiusOffset;
+  double P1, P2, P3, P4, P5;
+} pdgpu_gb_params;
+
+/* Component selection for pdgpu_compute / pdgpu_md. */
+#define PDGPU_NONBONDED 0x1u /* FF_NonBonded::calcForces */
+#define PDGPU_BONDED 0x2u    /* FF_Bonded bonds + angles + torsions */
+#define PDGPU_GB 0x4u        /* FF_GeneralizedBorn_Still::calcForces (FastMode) */
+
+/* Integrators of MolecularDynamics (md.h:76). */
+#define PDGPU_VELOCITY_VERLET 1
+#define PDGPU_LANGEVIN 3
+
+typedef struct pdgpu_md_params {
+  int32_t integrator;     /* PDGPU_VELOCITY_VERLET or PDGPU_LANGEVIN */
+  uint32_t components;    /* PDGPU_* forcefield components evaluated each step */
+  double timestep;        /* s (MolecularDynamics::Timestep) */
+  double temperature;     /* K (target temperature, Langevin only) */
+  double fric_coeff;      /* 1/s (MolecularDynamics::FricCoeff) */
+  int32_t update_nlist;   /* MolecularDynamics::UpdateNList (list rebuild period) */
+  int32_t step_offset;    /* first step number (Langevin's first/last step guards) */
+  int32_t total_steps;    /* MolecularDynamics::Steps */
+  uint64_t seed;          /* Langevin noise: Philox4x32-10 key */
+} pdgpu_md_params;
+
+/* ---- lifecycle -------------------------------------------------------------- */
+
+/* Creates an engine for n_atoms atoms on HIP device `device`. */
+int pdgpu_create(int32_t n_atoms, int32_t device, pdgpu_engine **out);
+int pdgpu_destroy(pdgpu_engine *e);
+const char *pdgpu_last_error(const pdgpu_engine *e);
+/* Library build string (gfx target, compile flags); never NULL. */
+const char *pdgpu_version(void);
+
+/* ---- system description (ForcefieldBase::setup time) ------------------------- */
+
+/* Per-atom Particle parameters (fundamentals.h:192-195) and masses. */
+int pdgpu_set_atom_params(pdgpu_engine *e, const double *radius, const double *epsilon,
+                          const double *charge, const double *mass);
+
+/* Molecules as inclusive atom ranges (MoleculeBase::mol[i].ifirst/ilast), used
+ * by WorkSpace::cleanSpace before each list rebuild. */
+int pdgpu_set_molecules(pdgpu_engine *e, int32_t n_mol, const int32_t *first,
+                        const int32_t *last);
+
+/* BondOrder data (bondorder.h:60-124): the band matrix getDiagData() of
+ * max_index_delta*(n-1) chars, and the off-diagonal partners of getOffDiagData()
+ * flattened as (i, j, order) triples in PD's order. */
+int pdgpu_set_bond_order(pdgpu_engine *e, int32_t max_index_delta, const int8_t *band,
+                         int32_t n_sparse, const int32_t *sp_i, const int32_t *sp_j,
+                         const int32_t *sp_order);
+
+/* Convenience: BondOrder::reinit (bondorder.cpp:94-212) from covalent 1-2/1-3/1-4
+ * partner lists in CSR form and residue indices, then pdgpu_set_bond_order. */
+int pdgpu_set_bond_order_from_cov(pdgpu_engine *e, const int32_t *resid,
+                                  const int32_t *off12, const int32_t *idx12,
+                                  const int32_t *off13, const int32_t *idx13,
+                                  const int32_t *off14, const int32_t *idx14);
+
+/* Space: periodic != 0 gives an orthogonal PeriodicBox of edge lengths box[3]
+ * centred on the origin (space.h:183-327); periodic == 0 gives InfiniteSpace. */
+int pdgpu_set_space(pdgpu_engine *e, int32_t periodic, const double *box);
+
+/* NeighbourListBase::requestCutoff / setPadding / calcShadow (neighbourlist.h:64-70). */
+int pdgpu_request_cutoff(pdgpu_engine *e, double cutoff);
+int pdgpu_set_padding(pdgpu_engine *e, double padding);
+int pdgpu_calc_shadow(pdgpu_engine *e, int32_t enabled);
+
+/* FF_NonBonded::setup (nonbonded.cpp:1025-1053): stores the parameters and
+ * requests max(Cutoff, VdwCutoff) from the list.  Fails with PDGPU_ERR_SETUP
+ * where FF_NonBonded::setupBasisVectors throws (more than 32 image vectors). */
+int pdgpu_setup_nonbonded(pdgpu_engine *e, const pdgpu_nb_params *p);
+
+/* FF_Bonded terms; force constants in PD units (J/A^2, J/rad^2, J), angles in rad.
+ * Torsions carry up to 4 Fourier terms each (vn/nn/gamma are 4 per torsion). */
+int pdgpu_set_bonded(pdgpu_engine *e, int32_t nb, const int32_t *bi, const int32_t *bj,
+                     const double *bl, const double *bk, int32_t na, const int32_t *ai,
+                     const int32_t *aa, const int32_t *aj, const double *theta0,
+                     const double *ak, int32_t nt, const int32_t *ti, const int32_t *ta,
+                     const int32_t *tb, const int32_t *tj, const int32_t *terms,
+                     const double *vn, const double *nn, const double *gamma);
+
+/* FF_GeneralizedBorn_Still::setup (gbff.cpp:57-104): GB radius of each atom's
+ * type and the 1-2 / 1-3 partner lists with their equilibrium distances
+ * (Particle::cov12atom/cov13atom, fundamentals.h:80-90).  The fixed Born terms
+ * (calcFixedBornRadiiTerms, gbff.cpp:324-398) are computed here in FP64. */
+int pdgpu_setup_gb(pdgpu_engine *e, const pdgpu_gb_params *p, const double *gbradius,
+                   const int32_t *off12, const int32_t *idx12, const double *d12,
+                   const int32_t *off13, const int32_t *idx13, const double *d13);
+
+/* ---- state ------------------------------------------------------------------ */
+
+int pdgpu_set_positions(pdgpu_engine *e, const double *xyz, size_t stride_bytes);
+int pdgpu_get_positions(pdgpu_engine *e, double *xyz, size_t stride_bytes);
+int pdgpu_set_velocities(pdgpu_engine *e, const double *xyz, size_t stride_bytes);
+int pdgpu_get_velocities(pdgpu_engine *e, double *xyz, size_t stride_bytes);
+
+/* ---- neighbour list ----------------------------------------------------------- */
+
+/* WorkSpace::cleanSpace then NeighbourList::calcNewList on the device
+ * (protocolbase.cpp:135-144).  Positions held by the engine are moved the way
+ * cleanSpace moves PD's. */
+int pdgpu_build_list(pdgpu_engine *e);
+
+/* NeighbourListBase::getFullUpdateCount (neighbourlist.h:71). */
+int pdgpu_full_update_count(pdgpu_engine *e, uint32_t *count);
+
+/* The pair set of the last build in PD's packed form (neighbourlist.h:131-151):
+ * per-atom counts and entries j | bondorder<<24 | image<<27, atom after atom,
+ * ascending j within an atom.  pdgpu_get_pair_count gives the total first. */
+int pdgpu_get_pair_count(pdgpu_engine *e, int64_t *total);
+int pdgpu_get_pairs(pdgpu_engine *e, int32_t *counts, uint32_t *packed, int64_t capacity);
+
+/* ---- force evaluation ---------------------------------------------------------- */
+
+/* Evaluates the selected components on the current positions, ADDS the forces
+ * (J/m) into f_accum (stride as above; may be NULL) and writes the components'
+ * energies (J) to *energies (may be NULL). */
+int pdgpu_compute(pdgpu_engine *e, uint32_t components, double *f_accum, size_t stride_bytes,
+                  pdgpu_energies *energies);
+
+/* Born radii of the last GB evaluation (calcBornRadii_PairwiseApprox), Angstrom. */
+int pdgpu_get_born_radii(pdgpu_engine *e, double *radii);
+
+/* ---- molecular dynamics ----------------------------------------------------------- */
+
+/* Runs n_steps steps of MolecularDynamics::run_core (md.cpp:219-284) on the
+ * device: list refresh every update_nlist steps, force evaluation, integrator.
+ * If epot_trace / ekin_trace are non-NULL they receive one value per step (J):
+ * the potential energy of the step's force evaluation and the kinetic energy the
+ * integrator computes.  Returns PDGPU_ERR_UNSTABLE if an energy became NaN. */
+int pdgpu_md(pdgpu_engine *e, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
+             double *ekin_trace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

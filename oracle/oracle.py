@@ -1,0 +1,343 @@
+"""ctypes front end of the FP64 oracle (libpd_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg.  Each wrapper names the PD function it restates;
+the C file cites PD's file:line for each.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpd_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+
+ENERGY_FIELDS = ("epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross",
+                 "epot_bond", "epot_angle", "epot_torsion", "epot_surf", "virial", "ekin")
+
+
+class Energies(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ENERGY_FIELDS]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in ENERGY_FIELDS}
+
+
+class NBParams(C.Structure):
+    """FF_NonBonded parameters (nonbonded.h:72-143); defaults of settodefault (nonbonded.cpp:985-1021)."""
+    _fields_ = [("Cutoff", C.c_double), ("InnerCutoff", C.c_double), ("VdwCutoff", C.c_double),
+                ("VdwInnerCutoff", C.c_double), ("Vdw14Scaling", C.c_double),
+                ("Elec14Scaling", C.c_double), ("Dielectric", C.c_double),
+                ("DDDielectricAlpha", C.c_double), ("DoVdw", C.c_int), ("DoElec", C.c_int),
+                ("ForceSwitch", C.c_int), ("EnergySwitch", C.c_int), ("VdwCor", C.c_int),
+                ("VdwCorDensity", C.c_double), ("VdwCorEpsilon", C.c_double),
+                ("VdwCorRadius", C.c_double), ("VdwCorCutoff", C.c_double)]
+
+
+class GBParams(C.Structure):
+    """FF_GeneralizedBorn_Still parameters (gbff.cpp:34-55)."""
+    _fields_ = [("Cutoff", C.c_double), ("InnerCutoff", C.c_double), ("VdwCutoff", C.c_double),
+                ("VdwInnerCutoff", C.c_double), ("Vdw14Scaling", C.c_double),
+                ("Elec14Scaling", C.c_double), ("DielectricSolute", C.c_double),
+                ("DielectricSolvent", C.c_double), ("GbsaStillCutoff", C.c_double),
+                ("ExpApproxThreshold", C.c_double), ("DielectricOffset", C.c_double),
+                ("BornRadiusOffset", C.c_double), ("P1", C.c_double), ("P2", C.c_double),
+                ("P3", C.c_double), ("P4", C.c_double), ("P5", C.c_double)]
+
+
+def nb_params(**kw) -> NBParams:
+    p = NBParams(Cutoff=15.0, InnerCutoff=12.0, VdwCutoff=15.0, VdwInnerCutoff=12.0,
+                 Vdw14Scaling=1.0, Elec14Scaling=1.0, Dielectric=1.0, DDDielectricAlpha=1.0,
+                 DoVdw=1, DoElec=1, ForceSwitch=1, EnergySwitch=0, VdwCor=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def gb_params(**kw) -> GBParams:
+    p = GBParams(Cutoff=15.0, InnerCutoff=12.0, VdwCutoff=15.0, VdwInnerCutoff=12.0,
+                 Vdw14Scaling=1.0, Elec14Scaling=1.0, DielectricSolute=1.0,
+                 DielectricSolvent=80.0, GbsaStillCutoff=10.0, ExpApproxThreshold=-4.0,
+                 DielectricOffset=-0.090, BornRadiusOffset=0.0, P1=0.073, P2=0.921, P3=6.211,
+                 P4=15.236, P5=1.254)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.pdo_constants.argtypes = [_dp]
+        L.pdo_basis_vectors.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.pdo_basis_vectors.restype = C.c_int
+        L.pdo_clean_space.argtypes = [_dp, C.c_int, _ip, _ip, _dp]
+        L.pdo_bondorder_mid.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip]
+        L.pdo_bondorder_mid.restype = C.c_int
+        L.pdo_bondorder_fill.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip,
+                                         _bp, _ip, _ip, _ip, C.c_int]
+        L.pdo_bondorder_fill.restype = C.c_int
+        L.pdo_build_list.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, _bp, C.c_int, _ip, _ip, _ip, _ip, _up, C.c_int64]
+        L.pdo_build_list.restype = C.c_int64
+        L.pdo_nb_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _up, _dp,
+                                    C.POINTER(NBParams), _dp, C.POINTER(Energies)]
+        L.pdo_nb_abs_energy.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _up, _dp, C.POINTER(NBParams)]
+        L.pdo_nb_abs_energy.restype = C.c_double
+        L.pdo_gb_fixed_terms.argtypes =[C.c_int, _dp, C.POINTER(GBParams), _ip, _ip, _dp, _ip, _ip,
+                                         _dp, _dp, _dp, _dp]
+        L.pdo_gb_radii.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.POINTER(GBParams), _ip, _up, _dp]
+        L.pdo_gb_forces_fast.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                         C.POINTER(GBParams), _ip, _up, _dp, C.POINTER(Energies),
+                                         C.c_void_p]
+        L.pdo_gb_energies.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.POINTER(GBParams), _ip,
+                                      _up, C.POINTER(Energies)]
+        L.pdo_bond_forces.argtypes = [_dp, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(Energies)]
+        L.pdo_angle_forces.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.POINTER(Energies)]
+        L.pdo_torsion_forces.argtypes = [_dp, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
+                                         C.POINTER(Energies)]
+        L.pdo_kinetic_energy.argtypes = [C.c_int, _dp, _dp]
+        L.pdo_kinetic_energy.restype = C.c_double
+        L.pdo_vv_step.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp, _dp]
+        L.pdo_vv_step.restype = C.c_double
+        L.pdo_langevin_step.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_int, _dp, _dp, _dp, _dp]
+        L.pdo_langevin_step.restype = C.c_double
+        L.pdo_langevin_coeffs.argtypes = [C.c_double, _dp]
+        L.pdo_remd_accept.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.pdo_remd_accept.restype = C.c_int
+        L.pdo_philox4x32.argtypes = [_up, _up, _up]
+        L.pdo_box_muller.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.pdo_langevin_noise.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _dp]
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# --------------------------------------------------------------------------- wrappers
+
+def constants() -> dict:
+    out = np.zeros(16)
+    lib().pdo_constants(out)
+    names = ["kcal2J", "J2kcal", "Na", "kcal2JDivNa", "kB", "R_kcal", "amu", "Angstrom",
+             "invAngstrom", "e_charge", "e0", "_4pi_e0", "econv", "halfeconv", "econv_joule",
+             "halfeconv_joule"]
+    return dict(zip(names, out.tolist()))
+
+
+def basis_vectors(periodic: bool, box, cutoff: float) -> np.ndarray:
+    out = np.zeros((32, 3))
+    b = _f64(box if box is not None else [0.0, 0.0, 0.0])
+    n = lib().pdo_basis_vectors(int(periodic), b, float(cutoff), out)
+    if n < 0:
+        raise ValueError("more than 32 periodic basis vectors (cutoff > half box)")
+    return out
+
+
+def clean_space(sys, pos=None) -> np.ndarray:
+    """WorkSpace::cleanSpace on a copy of pos (or sys.pos)."""
+    p = _f64(sys.pos if pos is None else pos).copy()
+    if sys.box is not None:
+        lib().pdo_clean_space(_f64(sys.box), len(sys.mol_first), _i32(sys.mol_first),
+                              _i32(sys.mol_last), p.reshape(-1))
+    return p
+
+
+@dataclass
+class BondOrderData:
+    mid: int
+    band: np.ndarray        # int8, mid*(n-1)
+    sp_i: np.ndarray
+    sp_j: np.ndarray
+    sp_order: np.ndarray
+
+
+def bond_order(sys, max_bond_order: int = 7) -> BondOrderData:
+    """BondOrder::reinit (bondorder.cpp:170-212)."""
+    c = sys.cov
+    L = lib()
+    mid = L.pdo_bondorder_mid(sys.n, _i32(sys.resid), c["off12"], c["idx12"], c["off13"],
+                              c["idx13"], c["off14"], c["idx14"])
+    band = np.zeros(max(mid * (sys.n - 1), 1), dtype=np.int8)
+    cap = max(16, len(c["idx12"]) + len(c["idx13"]) + len(c["idx14"]))
+    si = np.zeros(cap, np.int32); sj = np.zeros(cap, np.int32); so = np.zeros(cap, np.int32)
+    ns = L.pdo_bondorder_fill(sys.n, mid, max_bond_order, c["off12"], c["idx12"], c["off13"],
+                              c["idx13"], c["off14"], c["idx14"], band, si, sj, so, cap)
+    assert ns >= 0
+    return BondOrderData(mid, band, si[:ns].copy(), sj[:ns].copy(), so[:ns].copy())
+
+
+def build_list(sys, pos, cutoff: float, padding: float, calc_shadow: bool, bo: BondOrderData):
+    """NeighbourList::calcNewList (neighbourlist.cpp:222-244).  Returns (counts, packed)."""
+    n = sys.n
+    periodic = sys.box is not None
+    box = _f64(sys.box if periodic else [0, 0, 0])
+    counts = np.zeros(n, np.int32)
+    cap = 1 << 16
+    L = lib()
+    p = _f64(pos).reshape(-1)
+    sp_n = len(bo.sp_i)
+    spi = bo.sp_i if sp_n else np.zeros(1, np.int32)
+    spj = bo.sp_j if sp_n else np.zeros(1, np.int32)
+    spo = bo.sp_order if sp_n else np.zeros(1, np.int32)
+    while True:
+        packed = np.zeros(cap, np.uint32)
+        tot = L.pdo_build_list(n, p, int(periodic), box, float(cutoff), float(padding),
+                               int(calc_shadow), bo.mid, bo.band, sp_n, spi, spj, spo, counts,
+                               packed, cap)
+        if tot >= 0:
+            return counts, packed[:tot].copy()
+        cap = int(-tot)
+
+
+def nb_forces(sys, pos, counts, packed, params: NBParams, basis: np.ndarray):
+    """FF_NonBonded::calcForces -> fast2 kernel (nonbonded.cpp:559-910). Returns (f[J/m], energies)."""
+    f = np.zeros(3 * sys.n)
+    e = Energies()
+    lib().pdo_nb_forces(sys.n, _f64(pos).reshape(-1), _f64(sys.radius), _f64(sys.epsilon),
+                        _f64(sys.charge), counts, packed, _f64(basis).reshape(-1), C.byref(params),
+                        f, C.byref(e))
+    return f.reshape(-1, 3), e.as_dict()
+
+
+def nb_abs_energy(sys, pos, counts, packed, params: NBParams, basis: np.ndarray) -> float:
+    """Sum of |E_vdw| + |E_elec| over the evaluated pairs (J), the condition scale of E."""
+    return lib().pdo_nb_abs_energy(sys.n, _f64(pos).reshape(-1), _f64(sys.radius), _f64(sys.epsilon),
+                                   _f64(sys.charge), counts, packed, _f64(basis).reshape(-1),
+                                   C.byref(params))
+
+
+def gb_fixed_terms(sys, params: GBParams):
+    c = sys.cov
+    n = sys.n
+    Vj = np.zeros(n); t123 = np.zeros(n); br0 = np.zeros(n)
+    d12 = c["d12"] if len(c["d12"]) else np.zeros(1)
+    d13 = c["d13"] if len(c["d13"]) else np.zeros(1)
+    i12 = c["idx12"] if len(c["idx12"]) else np.zeros(1, np.int32)
+    i13 = c["idx13"] if len(c["idx13"]) else np.zeros(1, np.int32)
+    lib().pdo_gb_fixed_terms(n, _f64(sys.gbrad), C.byref(params), c["off12"], i12, _f64(d12),
+                             c["off13"], i13, _f64(d13), Vj, t123, br0)
+    return Vj, t123, br0
+
+
+def gb_radii(sys, pos, Vj, t123, params, counts, packed):
+    br = np.zeros(sys.n)
+    lib().pdo_gb_radii(sys.n, _f64(pos).reshape(-1), _f64(sys.gbrad), Vj, t123, C.byref(params),
+                       counts, packed, br)
+    return br
+
+
+def gb_forces_fast(sys, pos, Vj, born, params, counts, packed):
+    f = np.zeros(3 * sys.n)
+    e = Energies()
+    lib().pdo_gb_forces_fast(sys.n, _f64(pos).reshape(-1), _f64(sys.radius), _f64(sys.epsilon),
+                             _f64(sys.charge), _f64(sys.gbrad), Vj, _f64(born), C.byref(params),
+                             counts, packed, f, C.byref(e), None)
+    return f.reshape(-1, 3), e.as_dict()
+
+
+def gb_energies(sys, pos, born, params, counts, packed):
+    e = Energies()
+    lib().pdo_gb_energies(sys.n, _f64(pos).reshape(-1), _f64(sys.radius), _f64(sys.epsilon),
+                          _f64(sys.charge), _f64(born), C.byref(params), counts, packed, C.byref(e))
+    return e.as_dict()
+
+
+def bonded_forces(sys, pos):
+    """FF_Bonded bonds + angles + torsions (ffbonded.cpp:945-1177, 1531-1555)."""
+    f = np.zeros(3 * sys.n)
+    e = Energies()
+    p = _f64(pos).reshape(-1)
+    L = lib()
+    if len(sys.bonds):
+        L.pdo_bond_forces(p, len(sys.bonds), _i32(sys.bonds[:, 0]), _i32(sys.bonds[:, 1]),
+                          _f64(sys.bond_l), _f64(sys.bond_k), f, C.byref(e))
+    if len(sys.angles):
+        L.pdo_angle_forces(p, len(sys.angles), _i32(sys.angles[:, 0]), _i32(sys.angles[:, 1]),
+                           _i32(sys.angles[:, 2]), _f64(sys.angle_t0), _f64(sys.angle_k), f,
+                           C.byref(e))
+    if len(sys.torsions):
+        t = sys.torsions
+        L.pdo_torsion_forces(p, len(t), _i32(t[:, 0]), _i32(t[:, 1]), _i32(t[:, 2]), _i32(t[:, 3]),
+                             _i32(sys.tor_terms), _f64(sys.tor_vn).reshape(-1),
+                             _f64(sys.tor_n).reshape(-1), _f64(sys.tor_gamma).reshape(-1), f,
+                             C.byref(e))
+    return f.reshape(-1, 3), e.as_dict()
+
+
+def kinetic_energy(mass, vel) -> float:
+    return lib().pdo_kinetic_energy(len(mass), _f64(mass), _f64(vel).reshape(-1))
+
+
+def vv_step(mass, dt, f, pos, vel) -> float:
+    """In-place PD velocity Verlet step (md.cpp:805-855); returns the kinetic energy."""
+    assert pos.flags.c_contiguous and vel.flags.c_contiguous
+    return lib().pdo_vv_step(len(mass), _f64(mass), float(dt), _f64(f).reshape(-1),
+                             pos.reshape(-1), vel.reshape(-1))
+
+
+def langevin_step(mass, dt, gamma, temp, step, steps, f, noise, pos, vel) -> float:
+    return lib().pdo_langevin_step(len(mass), _f64(mass), float(dt), float(gamma), float(temp),
+                                   int(step), int(steps), _f64(f).reshape(-1),
+                                   _f64(noise).reshape(-1), pos.reshape(-1), vel.reshape(-1))
+
+
+def langevin_coeffs(gammat: float) -> np.ndarray:
+    out = np.zeros(6)
+    lib().pdo_langevin_coeffs(float(gammat), out)
+    return out
+
+
+def remd_accept(e_lo, e_hi, t_lo, t_hi, u) -> bool:
+    return bool(lib().pdo_remd_accept(e_lo, e_hi, t_lo, t_hi, u))
+
+
+def philox4x32(ctr, key) -> np.ndarray:
+    out = np.zeros(4, np.uint32)
+    lib().pdo_philox4x32(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def box_muller(u1, u2):
+    a = C.c_double(); b = C.c_double()
+    lib().pdo_box_muller(u1, u2, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def langevin_noise(seed, stream, step, atom) -> np.ndarray:
+    out = np.zeros(6)
+    lib().pdo_langevin_noise(seed, stream, step, atom, out)
+    return out
+
+
+def decode(packed):
+    """Split PD's packed entries into (j, bondorder, image) (neighbourlist.h:131-133)."""
+    p = np.asarray(packed, dtype=np.uint32)
+    return (p & 0x00FFFFFF).astype(np.int64), ((p >> 24) & 7).astype(np.int64), (p >> 27).astype(np.int64)

@@ -1,0 +1,1157 @@
+/*
+ * pd_oracle.c -- FP64 CPU restatement of PD's nonbonded / GB-SA hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine (see pd_oracle.h).
+ * Built with -O2 -ffp-contract=off so that every expression is evaluated in the
+ * order it is written, which is PD's order (file:line cited per function).
+ */
+#include "pd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants: forcefield.cpp:35-55, maths.cpp:23-25 ---------------------- */
+#define PD_PI 3.14159265358979323846
+static const double kcal2J = 4184.0;
+static const double Na = 6.0221418E23;
+static const double kB = 1.38065E-23;
+static const double amu = 1.66053878E-27;
+static const double Angstrom = 1E-10;
+static const double e_charge = 1.60217645E-19;
+static const double e0 = 8.854187817E-12;
+
+static double J2kcal(void) { return 1.0 / kcal2J; }
+static double econv_joule(void) {
+  const double four_pi = PD_PI * 4.0;
+  return (e_charge * e_charge) / ((four_pi * e0) * Angstrom);
+}
+static double econv_kcal(void) {
+  const double four_pi = PD_PI * 4.0;
+  return (J2kcal() * Na * (e_charge * e_charge)) / ((four_pi * e0) * Angstrom);
+}
+static double halfeconv(void) { return econv_kcal() * 0.5; }
+static double halfeconv_joule(void) { return econv_joule() * 0.5; }
+
+static inline double sqr(double x) { return x * x; }
+static inline double cube(double x) { return x * x * x; }
+
+void pdo_constants(double out[16]) {
+  const double four_pi = PD_PI * 4.0;
+  out[0] = kcal2J;
+  out[1] = J2kcal();
+  out[2] = Na;
+  out[3] = kcal2J / Na;
+  out[4] = kB;
+  out[5] = kB * Na * J2kcal();
+  out[6] = amu;
+  out[7] = Angstrom;
+  out[8] = 1.0 / Angstrom;
+  out[9] = e_charge;
+  out[10] = e0;
+  out[11] = four_pi * e0;
+  out[12] = econv_kcal();
+  out[13] = halfeconv();
+  out[14] = econv_joule();
+  out[15] = halfeconv_joule();
+}
+
+/* ---- PeriodicBox::setupCellDimensions, space.cpp:47-110 -------------------- */
+int pdo_basis_vectors(int periodic, const double box[3], double cutoff, double basis[32][3]) {
+  if (!periodic) { /* Space::getBasisVector default, space.h:60-61 */
+    basis[0][0] = basis[0][1] = basis[0][2] = 0.0;
+    return 1;
+  }
+  double half[3] = {box[0] / 2.0, box[1] / 2.0, box[2] / 2.0};
+  unsigned nx = (unsigned)ceil(cutoff / half[0]);
+  unsigned ny = (unsigned)ceil(cutoff / half[1]);
+  unsigned nz = (unsigned)ceil(cutoff / half[2]);
+  /* PD resizes to (2nx+1)(2ny+1)(2ny+1) (space.cpp:92) and fills i,j,k in
+   * [-1, n] (space.cpp:94-109); FF_NonBonded refuses more than 32 (nonbonded.cpp:1059). */
+  long count = (long)(2 * nx + 1) * (2 * ny + 1) * (2 * ny + 1);
+  if (count > 32) return -1;
+  int c = 0;
+  for (int i = -1; i <= (int)nx; i++)
+    for (int j = -1; j <= (int)ny; j++)
+      for (int k = -1; k <= (int)nz; k++) {
+        if (c < 32) {
+          basis[c][0] = (double)i * box[0];
+          basis[c][1] = (double)j * box[1];
+          basis[c][2] = (double)k * box[2];
+        }
+        c++;
+      }
+  return (int)count;
+}
+
+/* ---- WorkSpace::cleanSpace, workspace.cpp:371-380 --------------------------- */
+void pdo_clean_space(const double box[3], int n_mol, const int *mol_first, const int *mol_last,
+                     double *pos) {
+  double half[3] = {box[0] / 2.0, box[1] / 2.0, box[2] / 2.0};
+  for (int m = 0; m < n_mol; m++) {
+    /* MoleculeBase::getCentreOfGeometry(ifirst, ilast), molecule.cpp:850-858 */
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int i = mol_first[m]; i <= mol_last[m]; i++) {
+      c[0] += pos[3 * i + 0];
+      c[1] += pos[3 * i + 1];
+      c[2] += pos[3 * i + 2];
+    }
+    double cnt = (double)(mol_last[m] - mol_first[m] + 1);
+    c[0] /= cnt;
+    c[1] /= cnt;
+    c[2] /= cnt;
+    /* PeriodicBox::getClosestImage, space.h:215-227 */
+    double b[3] = {c[0], c[1], c[2]};
+    for (int d = 0; d < 3; d++) {
+      while (b[d] > half[d]) b[d] -= box[d];
+      while (b[d] < -half[d]) b[d] += box[d];
+    }
+    double s[3] = {b[0] - c[0], b[1] - c[1], b[2] - c[2]};
+    for (int i = mol_first[m]; i <= mol_last[m]; i++) {
+      pos[3 * i + 0] += s[0];
+      pos[3 * i + 1] += s[1];
+      pos[3 * i + 2] += s[2];
+    }
+  }
+}
+
+/* ---- BondOrder, bondorder.cpp:94-212 ----------------------------------------- */
+int pdo_bondorder_mid(int n, const int *resid, const int *off12, const int *idx12,
+                      const int *off13, const int *idx13, const int *off14, const int *idx14) {
+  int mid = 0;
+  const int *offs[3] = {off12, off13, off14};
+  const int *idxs[3] = {idx12, idx13, idx14};
+  for (int i = 0; i < n; i++)
+    for (int l = 0; l < 3; l++)
+      for (int k = offs[l][i]; k < offs[l][i + 1]; k++) {
+        int lookup = idxs[l][k];
+        int d = abs(i - lookup);
+        int rd = abs(resid[i] - resid[lookup]);
+        if (rd > 1) continue;
+        if (d > mid) mid = d;
+      }
+  return mid + 1;
+}
+
+int pdo_bondorder_fill(int n, int mid, int max_bond_order, const int *off12, const int *idx12,
+                       const int *off13, const int *idx13, const int *off14, const int *idx14,
+                       signed char *band, int *sp_i, int *sp_j, int *sp_order, int sp_cap) {
+  for (long k = 0; k < (long)mid * (n - 1); k++) band[k] = (signed char)max_bond_order;
+  /* PD keeps one LongBondOrder per i (getLongBondOrder, bondorder.cpp:152-167)
+   * and appends partners; the flat output is grouped by i in first-seen order. */
+  int ns = 0;
+  const int *offs[3] = {off14, off13, off12};
+  const int *idxs[3] = {idx14, idx13, idx12};
+  const int orders[3] = {3, 2, 1};
+  /* Entries are appended in calcOrder call order (14, 13, 12 per atom i).
+   * Because every i is handled completely before i+1, grouping by i in
+   * first-seen order equals this append order. */
+  for (int i = 0; i < n; i++)
+    for (int l = 0; l < 3; l++)
+      for (int k = offs[l][i]; k < offs[l][i + 1]; k++) {
+        int j = idxs[l][k];
+        if (j <= i) continue;
+        int delta = j - i;
+        int order = orders[l] < max_bond_order ? orders[l] : max_bond_order;
+        if (delta > mid) {
+          if (ns >= sp_cap) return -1;
+          sp_i[ns] = i;
+          sp_j[ns] = j;
+          sp_order[ns] = order;
+          ns++;
+        } else {
+          band[(long)i * mid + (delta - 1)] = (signed char)order;
+        }
+      }
+  return ns;
+}
+
+/* ---- NeighbourList, neighbourlist.cpp:179-439 -------------------------------- */
+static void correct_off_diagonal(int n_sparse, const int *sp_i, const int *sp_j,
+                                 const int *sp_order, int calc_shadow, const int64_t *start,
+                                 const int *counts, uint32_t *packed) {
+  /* correctOffDiagonalBondOrders, neighbourlist.cpp:179-220 */
+  for (int p = 0; p < n_sparse; p++) {
+    int a = sp_i[p], b = sp_j[p];
+    uint32_t bo = (uint32_t)(sp_order[p] & 7) << 24;
+    if (calc_shadow || a > b) {
+      for (int nj = 0; nj < counts[a]; nj++) {
+        uint32_t *e = &packed[start[a] + nj];
+        if ((int)(*e & 0x00FFFFFFu) == b) {
+          *e = (*e & 0xF8FFFFFFu) | bo;
+          break;
+        }
+      }
+    }
+    if (calc_shadow || a < b) {
+      for (int nj = 0; nj < counts[b]; nj++) {
+        uint32_t *e = &packed[start[b] + nj];
+        if ((int)(*e & 0x00FFFFFFu) == a) {
+          *e = (*e & 0xF8FFFFFFu) | bo;
+          break;
+        }
+      }
+    }
+  }
+}
+
+int64_t pdo_build_list(int n, const double *pos, int periodic, const double box[3], double cutoff,
+                       double padding, int calc_shadow, int mid, const signed char *band,
+                       int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
+                       int *counts, uint32_t *packed, int64_t cap) {
+  const double lim = sqr(cutoff + padding);
+  double half[3] = {0, 0, 0};
+  if (periodic) {
+    half[0] = box[0] / 2.0;
+    half[1] = box[1] / 2.0;
+    half[2] = box[2] / 2.0;
+  }
+  int64_t total = 0;
+  int64_t *start = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  for (int i = 0; i < n; i++) {
+    start[i] = total;
+    counts[i] = 0;
+    int limit = calc_shadow ? n : i;
+    for (int j = 0; j < limit; j++) {
+      double dx = pos[3 * j + 0] - pos[3 * i + 0];
+      double dy = pos[3 * j + 1] - pos[3 * i + 1];
+      double dz = pos[3 * j + 2] - pos[3 * i + 2];
+      int image = 13;
+      if (periodic) {
+        while (dx > half[0]) { dx -= box[0]; image -= 9; }
+        while (dx < -half[0]) { dx += box[0]; image += 9; }
+        while (dy > half[1]) { dy -= box[1]; image -= 3; }
+        while (dy < -half[1]) { dy += box[1]; image += 3; }
+        while (dz > half[2]) { dz -= box[2]; image -= 1; }
+        while (dz < -half[2]) { dz += box[2]; image += 1; }
+      }
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > lim) continue;
+      int bo = periodic ? 15 : 7;
+      if (abs(j - i) < mid) {
+        if (j > i)
+          bo = band[(long)i * mid + (j - i - 1)];
+        else
+          bo = band[(long)j * mid + (i - j - 1)];
+        if (j == i) continue;
+      }
+      uint32_t e = ((uint32_t)j & 0x00FFFFFFu) | ((uint32_t)(bo & 7) << 24);
+      if (periodic) e |= (uint32_t)(image & 31) << 27;
+      if (total < cap) packed[total] = e;
+      total++;
+      counts[i]++;
+    }
+  }
+  start[n] = total;
+  if (total > cap) {
+    free(start);
+    return -total;
+  }
+  correct_off_diagonal(n_sparse, sp_i, sp_j, sp_order, calc_shadow, start, counts, packed);
+  free(start);
+  return total;
+}
+
+/* ---- FF_NonBonded_CalcForces_T_fast2, nonbonded.cpp:559-910 ------------------ */
+static void nb_forces_impl(int n, const double *pos, const double *radius, const double *epsilon,
+                           const double *charge, const int *counts, const uint32_t *packed,
+                           const double basis[32][3], const pdo_nb_params *p, double *f,
+                           pdo_energies *e, double *abs_sum);
+
+void pdo_nb_forces(int n, const double *pos, const double *radius, const double *epsilon,
+                   const double *charge, const int *counts, const uint32_t *packed,
+                   const double basis[32][3], const pdo_nb_params *p, double *f,
+                   pdo_energies *e) {
+  nb_forces_impl(n, pos, radius, epsilon, charge, counts, packed, basis, p, f, e, 0);
+}
+
+double pdo_nb_abs_energy(int n, const double *pos, const double *radius, const double *epsilon,
+                         const double *charge, const int *counts, const uint32_t *packed,
+                         const double basis[32][3], const pdo_nb_params *p) {
+  double *f = (double *)calloc((size_t)n * 3, sizeof(double));
+  pdo_energies e;
+  memset(&e, 0, sizeof(e));
+  double s = 0.0;
+  nb_forces_impl(n, pos, radius, epsilon, charge, counts, packed, basis, p, f, &e, &s);
+  free(f);
+  return s;
+}
+
+static void nb_forces_impl(int n, const double *pos, const double *radius, const double *epsilon,
+                           const double *charge, const int *counts, const uint32_t *packed,
+                           const double basis[32][3], const pdo_nb_params *p, double *f,
+                           pdo_energies *e, double *abs_sum) {
+  /* dispatch, nonbonded.cpp:1228-1238: VdW always energy-switched when on */
+  int elec_mode = 1; /* normal */
+  if (p->EnergySwitch) elec_mode = 3;
+  if (p->ForceSwitch) elec_mode = 4;
+  if (!p->DoElec) elec_mode = 0;
+  int vdw_on = p->DoVdw ? 1 : 0;
+
+  const double sqrcutoff = sqr(p->Cutoff);
+  const double sqrinnercutoff = sqr(p->InnerCutoff);
+  const double invdielectric = 1.0 / p->Dielectric;
+  const double Swidth = p->Cutoff - p->InnerCutoff;
+  const double invSwidth = 1.0 / Swidth;
+  const double vdwSwidth = p->VdwCutoff - p->VdwInnerCutoff;
+  const double vdwinvSwidth = 1.0 / vdwSwidth;
+  const double C = p->Cutoff, I = p->InnerCutoff;
+  const double sA = 1.0 / cube(sqr(C) - sqr(I));
+  const double sB = -(cube(sqr(C)) - 3.0 * sqr(C) * sqr(C) * sqr(I));
+  const double sC = 6.0 * sqr(C) * sqr(I);
+  const double sD = -(sqr(C) + sqr(I));
+  const double sE = 2.0 / 5.0;
+  const double fswitch_innerV = sA * (sB * (1.0 / I) + sC * I + sD * cube(I) + sE * cube(I) * sqr(I));
+  const double fswitch_cutoffV = sA * (sB * (1.0 / C) + sC * C + sD * cube(C) + sE * cube(C) * sqr(C));
+  double eshift;
+  if (I < C)
+    eshift = 1 / I + (fswitch_innerV - fswitch_cutoffV);
+  else
+    eshift = 0;
+  const double tabVdw[8] = {0.0, 0.0, 0.0, p->Vdw14Scaling, 1.0, 1.0, 1.0, 1.0};
+  const double tabElec[8] = {0.0, 0.0, 0.0, p->Elec14Scaling, 1.0, 1.0, 1.0, 1.0};
+  const double ej = econv_joule();
+
+  e->epot_vdw = 0;
+  e->epot_elec = 0;
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const double ri = radius[i], epsi = epsilon[i], qi = charge[i];
+    double fix = 0, fiy = 0, fiz = 0;
+    const uint32_t *lst = packed + off;
+    off += counts[i];
+    for (int nj = 0; nj < counts[i]; nj++) {
+      int32_t jw = (int32_t)lst[nj];
+      int nbor_type = jw >> 24;
+      int j = jw & 0x00FFFFFF;
+      if (j >= i) break;
+      double vdw14scale = tabVdw[nbor_type & 7];
+      double elec14scale = tabElec[nbor_type & 7];
+      const double *bv = basis[(nbor_type >> 3) & 31];
+      double fx = pos[3 * j + 0] - pos[3 * i + 0];
+      double fy = pos[3 * j + 1] - pos[3 * i + 1];
+      double fz = pos[3 * j + 2] - pos[3 * i + 2];
+      fx += bv[0];
+      fy += bv[1];
+      fz += bv[2];
+      double sqrdistij = fx * fx + fy * fy + fz * fz;
+      if (sqrdistij > sqrcutoff) continue;
+      double Dist_ij = sqrt(sqrdistij);
+      double invdistij = 1.0 / Dist_ij;
+
+      double vdw_potential = 0, vdw_force = 0;
+      if (vdw_on) {
+        if (Dist_ij < p->VdwCutoff) {
+          double radiusij = ri + radius[j];
+          double eps = vdw14scale * epsi * epsilon[j];
+          double B = sqr(radiusij * invdistij);
+          B *= B * B;
+          double A = sqr(B);
+          vdw_potential = (2.0 * eps) * (0.5 * A - B);
+          vdw_force = -(12.0E10 * eps * invdistij) * (A - B);
+          if (Dist_ij > p->VdwInnerCutoff) {
+            double vdwS = (1.0 - sqr(vdwinvSwidth * (Dist_ij - p->VdwInnerCutoff)));
+            double vdwdSdd = -4.0E10 * sqr(vdwinvSwidth) * (Dist_ij - p->VdwInnerCutoff) * vdwS;
+            vdwS = sqr(vdwS);
+            vdw_force = vdwS * vdw_force + vdw_potential * vdwdSdd;
+            vdw_potential *= vdwS;
+          }
+        }
+      }
+
+      double elec_potential = 0, elec_force = 0;
+      if (elec_mode > 0) {
+        double qj = charge[j];
+        if (elec_mode == 1) {
+          elec_potential = ej * invdielectric * elec14scale * (qi * qj) * invdistij;
+          elec_force = -elec_potential * invdistij * 1E10;
+        } else if (elec_mode == 3) {
+          elec_potential = ej * invdielectric * elec14scale * (qi * qj) * invdistij;
+          elec_force = -elec_potential * invdistij * 1E10;
+          if (Dist_ij > p->InnerCutoff) {
+            double S = (1.0 - sqr(invSwidth * (Dist_ij - p->InnerCutoff)));
+            double dSdd = -4.0E10 * sqr(invSwidth) * (Dist_ij - p->InnerCutoff) * S;
+            S = sqr(S);
+            elec_force = S * elec_force + elec_potential * dSdd;
+            elec_potential *= S;
+          }
+        } else if (elec_mode == 4) {
+          elec_potential = ej * invdielectric * elec14scale * (qi * qj);
+          if (Dist_ij > p->InnerCutoff) {
+            elec_force = -1E10 * elec_potential * sA * sqr(invdistij) * sqr(sqrcutoff - sqrdistij) *
+                         (sqrcutoff - 3.0 * sqrinnercutoff + 2.0 * sqrdistij);
+            elec_potential *= -(sA * Dist_ij *
+                                    (sB * sqr(invdistij) + sC + sD * sqrdistij + sE * sqr(sqrdistij)) -
+                                fswitch_cutoffV);
+          } else {
+            elec_force = -1E10 * elec_potential * sqr(invdistij);
+            elec_potential *= (invdistij - eshift);
+          }
+        }
+      }
+
+      e->epot += vdw_potential + elec_potential;
+      e->epot_vdw += vdw_potential;
+      e->epot_elec += elec_potential;
+      if (abs_sum) *abs_sum += fabs(vdw_potential) + fabs(elec_potential);
+      double force_magnitude = vdw_force + elec_force;
+      e->virial += Dist_ij * force_magnitude * Angstrom;
+      double s = invdistij * force_magnitude;
+      fx *= s;
+      fy *= s;
+      fz *= s;
+      fix += fx;
+      fiy += fy;
+      fiz += fz;
+      f[3 * j + 0] -= fx;
+      f[3 * j + 1] -= fy;
+      f[3 * j + 2] -= fz;
+    }
+    f[3 * i + 0] += fix;
+    f[3 * i + 1] += fiy;
+    f[3 * i + 2] += fiz;
+    if (p->VdwCor) {
+      double R = p->VdwCorRadius;
+      double vdw_potential = -0.5 * p->VdwCorDensity * p->VdwCorEpsilon / (J2kcal() * Na) *
+                             (8.0 / 3.0) * PD_PI * sqr(R) * sqr(R) * sqr(R) / cube(p->VdwCorCutoff);
+      e->epot += vdw_potential;
+      e->epot_vdw += vdw_potential;
+    }
+  }
+}
+
+/* ---- GB Still fixed terms, gbff.cpp:324-398 ---------------------------------- */
+void pdo_gb_fixed_terms(int n, const double *gbrad, const pdo_gb_params *p, const int *off12,
+                        const int *idx12, const double *d12, const int *off13, const int *idx13,
+                        const double *d13, double *Vj, double *terms123, double *bornradius0) {
+  const double FourPI = 4.0 * PD_PI;
+  for (int j = 0; j < n; j++) {
+    double Rj = gbrad[j];
+    if (Rj <= 0.01) Rj = 1.15;
+    Vj[j] = FourPI * Rj * Rj * Rj / 3;
+    for (int nk = off12[j]; nk < off12[j + 1]; nk++) {
+      int k = idx12[nk];
+      double Rk = gbrad[k];
+      if (Rk <= 0.01) Rk = 1.15;
+      double rjk = 1.01 * d12[nk];
+      double hjk = Rj * (1.0 + (sqr(Rk) - sqr(Rj) - sqr(rjk)) / (2.0 * Rj * rjk));
+      Vj[j] -= PD_PI * sqr(hjk) * (3 * Rj - hjk) / 3;
+    }
+    if (Vj[j] < 0) Vj[j] = 0;
+  }
+  for (int i = 0; i < n; i++) {
+    double Rj = gbrad[i];
+    if (Rj <= 0.01) Rj = 1.15;
+    double term1 = -166.0 / (Rj + p->DielectricOffset + p->P1);
+    double term2 = 0;
+    for (int nj = off12[i]; nj < off12[i + 1]; nj++) {
+      int j = idx12[nj];
+      double rij = d12[nj];
+      term2 += p->P2 * Vj[j] / (sqr(rij) * sqr(rij));
+    }
+    double term3 = 0;
+    for (int nj = off13[i]; nj < off13[i + 1]; nj++) {
+      int j = idx13[nj];
+      double rij = d13[nj];
+      term3 += p->P3 * Vj[j] / (sqr(rij) * sqr(rij));
+    }
+    terms123[i] = term1 + term2 + term3;
+    if (bornradius0) bornradius0[i] = -halfeconv() / terms123[i];
+  }
+}
+
+/* ---- calcBornRadii_PairwiseApprox, gbff.cpp:432-483 -------------------------- */
+void pdo_gb_radii(int n, const double *pos, const double *gbrad, const double *Vj,
+                  const double *terms123, const pdo_gb_params *p, const int *counts,
+                  const uint32_t *packed, double *bornradius) {
+  const double invP5 = 1 / p->P5;
+  const double lim = sqr(p->GbsaStillCutoff);
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const double Ri = gbrad[i];
+    double Gpol = terms123[i];
+    const uint32_t *lst = packed + off;
+    off += counts[i];
+    for (int nj = 0; nj < counts[i]; nj++) {
+      int j = (int)(lst[nj] & 0x00FFFFFFu);
+      double gx = pos[3 * i + 0] - pos[3 * j + 0];
+      double gy = pos[3 * i + 1] - pos[3 * j + 1];
+      double gz = pos[3 * i + 2] - pos[3 * j + 2];
+      double Dist_ij = gx * gx + gy * gy + gz * gz; /* squared, as in PD */
+      if (Dist_ij > lim) continue;
+      if ((int)((lst[nj] >> 24) & 7) <= 2) continue;
+      double Rj = gbrad[j];
+      double rdivR = Dist_ij / sqr((Ri + Rj));
+      double CCF;
+      if (rdivR > invP5)
+        CCF = 1.0;
+      else
+        CCF = sqr(0.5 - 0.5 * cos((rdivR * p->P5 * PD_PI)));
+      Gpol += p->P4 * Vj[j] * CCF / (sqr(Dist_ij));
+    }
+    bornradius[i] = -halfeconv() / Gpol + p->BornRadiusOffset;
+  }
+}
+
+/* ---- calcForcesIncludingVacuo, gbff.cpp:1466-1835 ----------------------------- */
+void pdo_gb_forces_fast(int n, const double *pos, const double *radius, const double *epsilon,
+                        const double *charge, const double *gbrad, const double *Vj,
+                        const double *bornradius, const pdo_gb_params *p, const int *counts,
+                        const uint32_t *packed, double *f, pdo_energies *e, double *atom_epot) {
+  const double sqrcutoff = sqr(p->Cutoff);
+  const double Swidth = p->Cutoff - p->InnerCutoff;
+  const double invSwidth = 1.0 / Swidth;
+  const double vdwSwidth = p->VdwCutoff - p->VdwInnerCutoff;
+  const double vdwinvSwidth = 1.0 / vdwSwidth;
+  const double invdielectric = 1 / p->DielectricSolute;
+  const double invsolventdielectric = 1 / p->DielectricSolvent;
+  const double premul = -halfeconv_joule() * (1 / p->DielectricSolute - 1 / p->DielectricSolvent);
+  const double ej = econv_joule();
+
+  double *dedi = (double *)calloc((size_t)n * 3, sizeof(double));
+  double *deda = (double *)calloc((size_t)n, sizeof(double));
+  double epot_pol = 0, epot_pol_self = 0, epot_pol_cross = 0;
+  e->epot_vdw = 0;
+  e->epot_elec = 0;
+
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const double ai = bornradius[i];
+    const double atomix = pos[3 * i + 0], atomiy = pos[3 * i + 1], atomiz = pos[3 * i + 2];
+    double dedai = 0;
+    const double atomi_radius = radius[i], atomi_epsilon = epsilon[i], qi = charge[i];
+    const uint32_t *lst = packed + off;
+    off += counts[i];
+    for (int nj = 0; nj < counts[i]; nj++) {
+      int j = (int)(lst[nj] & 0x00FFFFFFu);
+      if (j > i) break;
+      int typej = (int)((lst[nj] >> 24) & 7);
+      double aj = bornradius[j];
+      double qiqj = qi * charge[j];
+      double dddx = atomix - pos[3 * j + 0];
+      double dddy = atomiy - pos[3 * j + 1];
+      double dddz = atomiz - pos[3 * j + 2];
+      double srij = sqr(dddx) + sqr(dddy) + sqr(dddz);
+      if (srij > sqrcutoff) continue;
+      double Dist_ij = sqrt(srij);
+      double invdistij = 1 / Dist_ij;
+      double aiaj = ai * aj;
+      double uGB = -srij * 0.25 / aiaj;
+      double pair_elec14scaling = 1.0;
+      if (typej == 3) pair_elec14scaling = p->Elec14Scaling;
+      double elec_potential = 0, elec_force = 0;
+      double dedx, dedy, dedz;
+      if (uGB > p->ExpApproxThreshold) {
+        double tGB = exp(uGB);
+        double mGB = tGB * aiaj;
+        double lGB = 1 / (srij + mGB);
+        double fGB = sqrt(lGB);
+        double epol = premul * qiqj * fGB;
+        double dedr = -epol * (Dist_ij - 0.25 * Dist_ij * tGB) * lGB;
+        double dedalpha = -epol * tGB * (1.0 - uGB) * lGB;
+        if (typej >= 3) {
+          elec_potential = ej * invdielectric * pair_elec14scaling * qiqj * invdistij;
+          elec_force = -elec_potential * invdistij;
+        }
+        if (Dist_ij >= p->InnerCutoff) {
+          double S = (1.0 - sqr(invSwidth * (Dist_ij - p->InnerCutoff)));
+          double dSdd = -4.0 * sqr(invSwidth) * (Dist_ij - p->InnerCutoff) * S;
+          S = sqr(S);
+          elec_force = S * elec_force + elec_potential * dSdd;
+          elec_potential *= S;
+          dedr = S * dedr + dSdd * epol;
+          dedalpha *= S;
+          epol *= S;
+        }
+        dedr *= 2 * invdistij;
+        dedx = dedr * dddx;
+        dedy = dedr * dddy;
+        dedz = dedr * dddz;
+        epot_pol += epol;
+        epot_pol += epol;
+        epot_pol_cross += epol;
+        epot_pol_cross += epol;
+        if (atom_epot) {
+          atom_epot[i] += epol;
+          atom_epot[j] += epol;
+        }
+        dedi[3 * i + 0] += dedx;
+        dedi[3 * i + 1] += dedy;
+        dedi[3 * i + 2] += dedz;
+        dedi[3 * j + 0] -= dedx;
+        dedi[3 * j + 1] -= dedy;
+        dedi[3 * j + 2] -= dedz;
+        dedai += dedalpha * aj;
+        deda[j] += dedalpha * ai;
+      } else {
+        if (typej >= 3) {
+          elec_potential = ej * invsolventdielectric * pair_elec14scaling * qiqj * invdistij;
+          elec_force = -elec_potential * invdistij;
+          if (Dist_ij >= p->InnerCutoff) {
+            double S = (1.0 - sqr(invSwidth * (Dist_ij - p->InnerCutoff)));
+            double dSdd = -4.0 * sqr(invSwidth) * (Dist_ij - p->InnerCutoff) * S;
+            S = sqr(S);
+            elec_force = S * elec_force + elec_potential * dSdd;
+            elec_potential *= S;
+          }
+        }
+      }
+      if (typej >= 3) {
+        double pair_vdw14scaling = 1.0;
+        if (typej == 3) pair_vdw14scaling = p->Vdw14Scaling;
+        double vdw_force = 0;
+        if (Dist_ij < p->VdwCutoff) {
+          double radiusij = sqr(invdistij * (atomi_radius + radius[j]));
+          double eps = pair_vdw14scaling * atomi_epsilon * epsilon[j];
+          double B = cube(radiusij);
+          double A = sqr(B);
+          double vdw_potential = (2.0 * eps) * (0.5 * A - B);
+          vdw_force = -(12.0 * eps * invdistij) * (A - B);
+          if (Dist_ij > p->VdwInnerCutoff) {
+            double vdwS = (1.0 - sqr(vdwinvSwidth * (Dist_ij - p->VdwInnerCutoff)));
+            double vdwdSdd = -4.0 * sqr(vdwinvSwidth) * (Dist_ij - p->VdwInnerCutoff) * vdwS;
+            vdwS = sqr(vdwS);
+            vdw_force = vdwS * vdw_force + vdw_potential * vdwdSdd;
+            vdw_potential *= vdwS;
+          }
+          e->epot += vdw_potential;
+          e->epot_vdw += vdw_potential;
+          if (atom_epot) {
+            atom_epot[i] += vdw_potential * 0.5;
+            atom_epot[j] += vdw_potential * 0.5;
+          }
+        }
+        e->epot += elec_potential;
+        e->epot_elec += elec_potential;
+        if (atom_epot) {
+          atom_epot[i] += elec_potential * 0.5;
+          atom_epot[j] += elec_potential * 0.5;
+        }
+        double force_magnitude = 0;
+        force_magnitude += vdw_force;
+        force_magnitude += elec_force;
+        force_magnitude *= invdistij;
+        dedx = force_magnitude * dddx;
+        dedy = force_magnitude * dddy;
+        dedz = force_magnitude * dddz;
+        dedi[3 * i + 0] += dedx;
+        dedi[3 * i + 1] += dedy;
+        dedi[3 * i + 2] += dedz;
+        dedi[3 * j + 0] -= dedx;
+        dedi[3 * j + 1] -= dedy;
+        dedi[3 * j + 2] -= dedz;
+      }
+    }
+    deda[i] += dedai;
+    double epol = premul * sqr(qi) / ai;
+    epot_pol += epol;
+    epot_pol_self += epol;
+    if (atom_epot) atom_epot[i] += epol;
+    deda[i] += -epol / ai;
+  }
+  e->epot += epot_pol;
+  e->epot_pol += epot_pol;
+  e->epot_pol_cross += epot_pol_cross;
+  e->epot_pol_self += epot_pol_self;
+
+  /* Born-radius chain rule, gbff.cpp:1768-1825 */
+  const double invP5 = 1 / p->P5;
+  const double lim = sqr(p->GbsaStillCutoff);
+  off = 0;
+  for (int i = 0; i < n; i++) {
+    const double Ri = gbrad[i];
+    const double gpi = -sqr(bornradius[i]) / -166.0;
+    const uint32_t *lst = packed + off;
+    off += counts[i];
+    for (int nj = 0; nj < counts[i]; nj++) {
+      int j = (int)(lst[nj] & 0x00FFFFFFu);
+      double dddx = (pos[3 * j + 0] - pos[3 * i + 0]);
+      double dddy = (pos[3 * j + 1] - pos[3 * i + 1]);
+      double dddz = (pos[3 * j + 2] - pos[3 * i + 2]);
+      double srij = sqr(dddx) + sqr(dddy) + sqr(dddz);
+      if (srij > lim) continue;
+      if ((int)((lst[nj] >> 24) & 7) <= 2) continue;
+      double Rj = gbrad[j];
+      double Vjj = Vj[j];
+      double r6 = srij * srij * srij;
+      double rdivR = srij / sqr((Ri + Rj));
+      double CCF, dCCF;
+      if (rdivR > invP5) {
+        CCF = 1.0;
+        dCCF = 0.0;
+      } else {
+        double theta = rdivR * p->P5 * PD_PI;
+        double sintheta = sin(theta);
+        double costheta = cos(theta);
+        double sqrtCCF = 0.5 - 0.5 * costheta;
+        CCF = sqr(sqrtCCF);
+        dCCF = 2.0 * sqrtCCF * sintheta * theta;
+      }
+      double dedr = gpi * deda[i] * p->P4 * Vjj * (4.0 * CCF - dCCF) / r6;
+      double dedx = dedr * dddx, dedy = dedr * dddy, dedz = dedr * dddz;
+      dedi[3 * i + 0] += dedx;
+      dedi[3 * i + 1] += dedy;
+      dedi[3 * i + 2] += dedz;
+      dedi[3 * j + 0] -= dedx;
+      dedi[3 * j + 1] -= dedy;
+      dedi[3 * j + 2] -= dedz;
+    }
+  }
+  const double conv = (double)-1.0 / (double)(Angstrom);
+  for (int i = 0; i < n; i++) {
+    f[3 * i + 0] += dedi[3 * i + 0] * conv;
+    f[3 * i + 1] += dedi[3 * i + 1] * conv;
+    f[3 * i + 2] += dedi[3 * i + 2] * conv;
+  }
+  free(dedi);
+  free(deda);
+}
+
+/* ---- energy-only GB, gbff.cpp:178-183 + 537-596 ------------------------------- */
+void pdo_gb_energies(int n, const double *pos, const double *radius, const double *epsilon,
+                     const double *charge, const double *bornradius, const pdo_gb_params *p,
+                     const int *counts, const uint32_t *packed, pdo_energies *e) {
+  /* FF_NonBonded::calcEnergies with the GB's settings: Dielectric =
+   * DielectricSolute (gbff.cpp:62), EnergySwitch on, ForceSwitch off (gbff.cpp:45-46). */
+  pdo_nb_params nb;
+  memset(&nb, 0, sizeof(nb));
+  nb.Cutoff = p->Cutoff;
+  nb.InnerCutoff = p->InnerCutoff;
+  nb.VdwCutoff = p->VdwCutoff;
+  nb.VdwInnerCutoff = p->VdwInnerCutoff;
+  nb.Vdw14Scaling = p->Vdw14Scaling;
+  nb.Elec14Scaling = p->Elec14Scaling;
+  nb.Dielectric = p->DielectricSolute;
+  nb.DoVdw = 1;
+  nb.DoElec = 1;
+  nb.EnergySwitch = 1;
+  nb.ForceSwitch = 0;
+  double basis[32][3] = {{0, 0, 0}};
+  double *scratch = (double *)calloc((size_t)n * 3, sizeof(double));
+  pdo_nb_forces(n, pos, radius, epsilon, charge, counts, packed, basis, &nb, scratch, e);
+  free(scratch);
+
+  const double bornpremul = -halfeconv_joule() * (1 / p->DielectricSolute - 1 / p->DielectricSolvent);
+  const double Swidth = p->Cutoff - p->InnerCutoff;
+  const double invSwidth = 1 / Swidth;
+  double epot_pol = 0, epot_pol_self = 0, epot_pol_cross = 0;
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const double qi = charge[i];
+    const double ai = bornradius[i];
+    epot_pol -= sqr(qi) / ai;
+    epot_pol_self -= sqr(qi) / ai;
+    const uint32_t *lst = packed + off;
+    off += counts[i];
+    for (int nj = 0; nj < counts[i]; nj++) {
+      int j = (int)(lst[nj] & 0x00FFFFFFu);
+      if (i > j) continue;
+      double gx = pos[3 * i + 0] - pos[3 * j + 0];
+      double gy = pos[3 * i + 1] - pos[3 * j + 1];
+      double gz = pos[3 * i + 2] - pos[3 * j + 2];
+      double Dist_ij = sqrt(gx * gx + gy * gy + gz * gz);
+      if (Dist_ij > p->Cutoff) continue;
+      double S;
+      if (Dist_ij < p->InnerCutoff) {
+        S = 1.0;
+      } else {
+        S = (1.0 - sqr(invSwidth * (Dist_ij - p->InnerCutoff)));
+        S = sqr(S);
+      }
+      double srij = sqr(Dist_ij);
+      double qj = charge[j];
+      double aiaj = ai * bornradius[j];
+      double epol = 2.0 * S * qi * qj / sqrt(srij + aiaj * exp(-srij * 0.25 / aiaj));
+      epot_pol -= epol;
+      epot_pol_cross -= epol;
+    }
+  }
+  epot_pol *= -bornpremul;
+  epot_pol_cross *= -bornpremul;
+  epot_pol_self *= -bornpremul;
+  e->epot += epot_pol;
+  e->epot_pol += epot_pol;
+  e->epot_pol_self += epot_pol_self;
+  e->epot_pol_cross += epot_pol_cross;
+}
+
+/* ---- FF_Bonded, ffbonded.cpp ------------------------------------------------- */
+void pdo_bond_forces(const double *pos, int nb, const int *bi, const int *bj, const double *bl,
+                     const double *bk, double *f, pdo_energies *e) {
+  /* calcBondForces, ffbonded.cpp:945-992 */
+  double epot_bond = 0;
+  for (int ib = 0; ib < nb; ib++) {
+    int i = bi[ib], j = bj[ib];
+    double k = bk[ib];
+    double vx = pos[3 * i + 0] - pos[3 * j + 0];
+    double vy = pos[3 * i + 1] - pos[3 * j + 1];
+    double vz = pos[3 * i + 2] - pos[3 * j + 2];
+    double d = sqrt(vx * vx + vy * vy + vz * vz);
+    double force_magnitude = 2 * k * (d - bl[ib]) / Angstrom;
+    e->virial += d * force_magnitude * Angstrom;
+    double potential = k * sqr(d - bl[ib]);
+    epot_bond += potential;
+    double s = force_magnitude / d;
+    vx *= s;
+    vy *= s;
+    vz *= s;
+    f[3 * i + 0] -= vx;
+    f[3 * i + 1] -= vy;
+    f[3 * i + 2] -= vz;
+    f[3 * j + 0] += vx;
+    f[3 * j + 1] += vy;
+    f[3 * j + 2] += vz;
+  }
+  e->epot_bond += epot_bond;
+  e->epot += epot_bond;
+}
+
+void pdo_angle_forces(const double *pos, int na, const int *ai_, const int *aa_, const int *aj_,
+                      const double *theta0, const double *ak, double *f, pdo_energies *e) {
+  /* calcAngleForces, ffbonded.cpp:1054-1177 */
+  double epot_angle = 0;
+  for (int ib = 0; ib < na; ib++) {
+    if (ak[ib] == 0) continue;
+    int i = ai_[ib], a = aa_[ib], j = aj_[ib];
+    double ivx = pos[3 * i + 0] - pos[3 * a + 0];
+    double ivy = pos[3 * i + 1] - pos[3 * a + 1];
+    double ivz = pos[3 * i + 2] - pos[3 * a + 2];
+    double iv_mag = sqrt(sqr(ivx) + sqr(ivy) + sqr(ivz));
+    double inv_iv_mag = 1 / iv_mag;
+    double uix = ivx * inv_iv_mag, uiy = ivy * inv_iv_mag, uiz = ivz * inv_iv_mag;
+    double jvx = pos[3 * j + 0] - pos[3 * a + 0];
+    double jvy = pos[3 * j + 1] - pos[3 * a + 1];
+    double jvz = pos[3 * j + 2] - pos[3 * a + 2];
+    double jv_mag = sqrt(sqr(jvx) + sqr(jvy) + sqr(jvz));
+    double inv_jv_mag = 1 / jv_mag;
+    double ujx = jvx * inv_jv_mag, ujy = jvy * inv_jv_mag, ujz = jvz * inv_jv_mag;
+    double cos_theta = uix * ujx + uiy * ujy + uiz * ujz;
+    double theta = acos(cos_theta);
+    double sin_theta = sin(theta);
+    double k = ak[ib];
+    double potential = k * sqr(theta - theta0[ib]);
+    epot_angle += potential;
+    double force_magnitude = 2 * k * (theta - theta0[ib]) / (Angstrom);
+    double bx = force_magnitude * inv_iv_mag * (uix * cos_theta - ujx) / sin_theta;
+    double by = force_magnitude * inv_iv_mag * (uiy * cos_theta - ujy) / sin_theta;
+    double bz = force_magnitude * inv_iv_mag * (uiz * cos_theta - ujz) / sin_theta;
+    f[3 * i + 0] -= bx;
+    f[3 * i + 1] -= by;
+    f[3 * i + 2] -= bz;
+    f[3 * a + 0] += bx;
+    f[3 * a + 1] += by;
+    f[3 * a + 2] += bz;
+    bx = force_magnitude * inv_jv_mag * (ujx * cos_theta - uix) / sin_theta;
+    by = force_magnitude * inv_jv_mag * (ujy * cos_theta - uiy) / sin_theta;
+    bz = force_magnitude * inv_jv_mag * (ujz * cos_theta - uiz) / sin_theta;
+    f[3 * j + 0] -= bx;
+    f[3 * j + 1] -= by;
+    f[3 * j + 2] -= bz;
+    f[3 * a + 0] += bx;
+    f[3 * a + 1] += by;
+    f[3 * a + 2] += bz;
+  }
+  e->epot_angle += epot_angle;
+  e->epot += epot_angle;
+}
+
+void pdo_torsion_forces(const double *pos, int nt, const int *ti_, const int *ta_,
+                        const int *tb_, const int *tj_, const int *terms, const double *Vn,
+                        const double *nn, const double *gamma, double *f, pdo_energies *e) {
+  /* calcDihedralForces<false,false>, ffbonded.cpp:1274-1512 */
+  double epot_torsion = 0;
+  for (int t = 0; t < nt; t++) {
+    int ti = ti_[t], ta = ta_[t], tb = tb_[t], tj = tj_[t];
+    double v1x = pos[3 * ti + 0] - pos[3 * ta + 0];
+    double v1y = pos[3 * ti + 1] - pos[3 * ta + 1];
+    double v1z = pos[3 * ti + 2] - pos[3 * ta + 2];
+    double v2x = pos[3 * ta + 0] - pos[3 * tb + 0];
+    double v2y = pos[3 * ta + 1] - pos[3 * tb + 1];
+    double v2z = pos[3 * ta + 2] - pos[3 * tb + 2];
+    double v3x = pos[3 * tb + 0] - pos[3 * tj + 0];
+    double v3y = pos[3 * tb + 1] - pos[3 * tj + 1];
+    double v3z = pos[3 * tb + 2] - pos[3 * tj + 2];
+    double fix = 0, fiy = 0, fiz = 0, fabx = 0, faby = 0, fabz = 0, fjx = 0, fjy = 0, fjz = 0;
+    double n1x = (v1y * v2z - v1z * v2y);
+    double n1y = (v1z * v2x - v1x * v2z);
+    double n1z = (v1x * v2y - v1y * v2x);
+    double n2x = (v2y * v3z - v2z * v3y);
+    double n2y = (v2z * v3x - v2x * v3z);
+    double n2z = (v2x * v3y - v2y * v3x);
+    double n3x = (v2y * n1z - v2z * n1y);
+    double n3y = (v2z * n1x - v2x * n1z);
+    double n3z = (v2x * n1y - v2y * n1x);
+    double in1 = 1.0 / sqrt(sqr(n1x) + sqr(n1y) + sqr(n1z));
+    double in2 = 1.0 / sqrt(sqr(n2x) + sqr(n2y) + sqr(n2z));
+    double in3 = 1.0 / sqrt(sqr(n3x) + sqr(n3y) + sqr(n3z));
+    double cos_phi = (n1x * n2x + n1y * n2y + n1z * n2z) * in1 * in2;
+    double sin_phi = (n3x * n2x + n3y * n2y + n3z * n2z) * in3 * in2;
+    n2x *= in2;
+    n2y *= in2;
+    n2z *= in2;
+    double phi = -atan2(sin_phi, cos_phi);
+    double dc1x = 0, dc1y = 0, dc1z = 0, dc2x = 0, dc2y = 0, dc2z = 0;
+    double ds3x = 0, ds3y = 0, ds3z = 0, ds2x = 0, ds2y = 0, ds2z = 0;
+    int use_cos = fabs(sin_phi) > 0.1;
+    if (use_cos) {
+      n1x *= in1;
+      n1y *= in1;
+      n1z *= in1;
+      dc1x = in1 * (n1x * cos_phi - n2x);
+      dc1y = in1 * (n1y * cos_phi - n2y);
+      dc1z = in1 * (n1z * cos_phi - n2z);
+      dc2x = in2 * (n2x * cos_phi - n1x);
+      dc2y = in2 * (n2y * cos_phi - n1y);
+      dc2z = in2 * (n2z * cos_phi - n1z);
+    } else {
+      n3x *= in3;
+      n3y *= in3;
+      n3z *= in3;
+      ds3x = in3 * (n3x * sin_phi - n2x);
+      ds3y = in3 * (n3y * sin_phi - n2y);
+      ds3z = in3 * (n3z * sin_phi - n2z);
+      ds2x = in2 * (n2x * sin_phi - n3x);
+      ds2y = in2 * (n2y * sin_phi - n3y);
+      ds2z = in2 * (n2z * sin_phi - n3z);
+    }
+    for (int k = 0; k < terms[t]; k++) {
+      double kk = Vn[4 * t + k], n = nn[4 * t + k], g = gamma[4 * t + k];
+      double ep, dEdphi;
+      if (n > 0) {
+        ep = kk * (1.0 + cos(n * phi + g));
+        dEdphi = -n * kk * sin(n * phi + g);
+      } else {
+        double diff = phi - g;
+        if (diff < -PD_PI)
+          diff += 2.0 * PD_PI;
+        else if (diff > PD_PI)
+          diff -= 2.0 * PD_PI;
+        ep = kk * sqr(diff);
+        dEdphi = 2.0 * kk * diff;
+      }
+      epot_torsion += ep;
+      if (use_cos) {
+        dEdphi /= sin_phi;
+        fix += dEdphi * (v2y * dc1z - v2z * dc1y);
+        fiy += dEdphi * (v2z * dc1x - v2x * dc1z);
+        fiz += dEdphi * (v2x * dc1y - v2y * dc1x);
+        fjx += dEdphi * (v2z * dc2y - v2y * dc2z);
+        fjy += dEdphi * (v2x * dc2z - v2z * dc2x);
+        fjz += dEdphi * (v2y * dc2x - v2x * dc2y);
+        fabx += dEdphi * (v1z * dc1y - v1y * dc1z + v3y * dc2z - v3z * dc2y);
+        faby += dEdphi * (v1x * dc1z - v1z * dc1x + v3z * dc2x - v3x * dc2z);
+        fabz += dEdphi * (v1y * dc1x - v1x * dc1y + v3x * dc2y - v3y * dc2x);
+      } else {
+        dEdphi /= -cos_phi;
+        fix += dEdphi * ((v2y * v2y + v2z * v2z) * ds3x - v2x * v2y * ds3y - v2x * v2z * ds3z);
+        fiy += dEdphi * ((v2z * v2z + v2x * v2x) * ds3y - v2y * v2z * ds3z - v2y * v2x * ds3x);
+        fiz += dEdphi * ((v2x * v2x + v2y * v2y) * ds3z - v2z * v2x * ds3x - v2z * v2y * ds3y);
+        fjx += dEdphi * (ds2y * v2z - ds2z * v2y);
+        fjy += dEdphi * (ds2z * v2x - ds2x * v2z);
+        fjz += dEdphi * (ds2x * v2y - ds2y * v2x);
+        fabx += dEdphi * (-(v2y * v1y + v2z * v1z) * ds3x + (2.0 * v2x * v1y - v1x * v2y) * ds3y +
+                          (2.0 * v2x * v1z - v1x * v2z) * ds3z + ds2z * v3y - ds2y * v3z);
+        faby += dEdphi * (-(v2z * v1z + v2x * v1x) * ds3y + (2.0 * v2y * v1z - v1y * v2z) * ds3z +
+                          (2.0 * v2y * v1x - v1y * v2x) * ds3x + ds2x * v3z - ds2z * v3x);
+        fabz += dEdphi * (-(v2x * v1x + v2y * v1y) * ds3z + (2.0 * v2z * v1x - v1z * v2x) * ds3x +
+                          (2.0 * v2z * v1y - v1z * v2y) * ds3y + ds2y * v3x - ds2x * v3y);
+      }
+    }
+    const double ia = 1.0 / Angstrom;
+    fix *= ia; fiy *= ia; fiz *= ia;
+    fabx *= ia; faby *= ia; fabz *= ia;
+    fjx *= ia; fjy *= ia; fjz *= ia;
+    f[3 * ti + 0] += fix;
+    f[3 * ti + 1] += fiy;
+    f[3 * ti + 2] += fiz;
+    f[3 * ta + 0] += fabx - fix;
+    f[3 * ta + 1] += faby - fiy;
+    f[3 * ta + 2] += fabz - fiz;
+    f[3 * tb + 0] += fjx - fabx;
+    f[3 * tb + 1] += fjy - faby;
+    f[3 * tb + 2] += fjz - fabz;
+    f[3 * tj + 0] -= fjx;
+    f[3 * tj + 1] -= fjy;
+    f[3 * tj + 2] -= fjz;
+  }
+  e->epot_torsion += epot_torsion;
+  e->epot += epot_torsion;
+}
+
+/* ---- MolecularDynamics, md.cpp ------------------------------------------------ */
+double pdo_kinetic_energy(int n, const double *mass, const double *vel) {
+  /* calcKineticEnergy, md.cpp:521-532 */
+  double ekin = 0;
+  for (int i = 0; i < n; i++) {
+    double m = sqrt(vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] +
+                    vel[3 * i + 2] * vel[3 * i + 2]);
+    ekin += (0.5 * mass[i] * sqr(m));
+  }
+  return ekin;
+}
+
+double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
+                   double *vel) {
+  /* applyForces_VelocityVerletIntegration, md.cpp:805-855 */
+  const double t = dt;
+  for (int i = 0; i < n; i++) {
+    double tinvmass = 0.5 * t / mass[i];
+    vel[3 * i + 0] += f[3 * i + 0] * tinvmass;
+    vel[3 * i + 1] += f[3 * i + 1] * tinvmass;
+    vel[3 * i + 2] += f[3 * i + 2] * tinvmass;
+  }
+  double ekin = pdo_kinetic_energy(n, mass, vel);
+  const double ia = 1 / Angstrom;
+  for (int i = 0; i < n; i++) {
+    double tinvmass = 0.5 * t / mass[i];
+    for (int d = 0; d < 3; d++) {
+      double dv = f[3 * i + d] * tinvmass;
+      vel[3 * i + d] += dv;
+      double dr = t * (vel[3 * i + d] + dv);
+      dr *= ia;
+      pos[3 * i + d] += dr;
+      vel[3 * i + d] += dv;
+    }
+  }
+  return ekin;
+}
+
+void pdo_langevin_coeffs(double gammat, double out[6]) {
+  double gt = gammat, gt2 = gt * gt, gt3 = gt * gt2, gt4 = gt2 * gt2, gt5 = gt2 * gt3,
+         gt6 = gt3 * gt3, gt7 = gt3 * gt4, gt8 = gt4 * gt4, gt9 = gt4 * gt5;
+  double c0, c1, c2, varr, varv, crv;
+  if (gammat > 0.05) {
+    c0 = exp(-gammat);
+    c1 = (1 - c0) / gammat;
+    c2 = (1 - c1) / gammat;
+    double egammat = exp(-gammat);
+    varv = (1.0 - sqr(egammat));
+    varr = (2.0 * gammat - 3.0 + (4.0 - egammat) * egammat);
+    crv = sqr(1.0 - egammat) / (sqrt(varv * varr));
+  } else {
+    c0 = 1 - gt + 0.5 * gt2 - gt3 / 6.0 + gt4 / 24.0 - gt5 / 120.0 + gt6 / 720.0 - gt7 / 5040.0 +
+         gt8 / 40320.0 - gt9 / 362880.0;
+    c1 = 1 - (0.5 * gt - gt2 / 6.0 + gt3 / 24.0 - gt4 / 120.0 + gt5 / 720.0 - gt6 / 5040.0 +
+              gt7 / 40320.0 - gt8 / 362880.0);
+    c2 = 0.5 - gt / 6.0 + gt2 / 24.0 - gt3 / 120.0 + gt4 / 720.0 - gt5 / 5040.0 + gt6 / 40320.0 -
+         gt7 / 362880.0;
+    varr = 2.0 * gt3 / 3.0 - gt4 / 2.0 + 7.0 * gt5 / 30.0 - gt6 / 12.0 + 31.0 * gt7 / 1260.0 -
+           gt8 / 160.0 + 127.0 * gt9 / 90720.0;
+    varv = 2.0 * gt - 2.0 * gt2 + 4.0 * gt3 / 3.0 - 2.0 * gt4 / 3.0 + 4.0 * gt5 / 15.0 -
+           4.0 * gt6 / 45.0 + 8.0 * gt7 / 315.0 - 2.0 * gt8 / 315.0 + 4.0 * gt9 / 2835.0;
+    crv = sqrt(3.0) * (0.5 - 3.0 * gt / 16.0 - 17.0 * gt2 / 1280.0 + 17.0 * gt3 / 6144.0 +
+                       40967.0 * gt4 / 34406400.0 - 57203.0 * gt5 / 275251200.0 -
+                       1429487.0 * gt6 / 13212057600.0);
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = varr;
+  out[4] = varv;
+  out[5] = crv;
+}
+
+double pdo_langevin_step(int n, const double *mass, double dt, double gamma, double temp,
+                         int step, int steps, const double *f, const double *noise, double *pos,
+                         double *vel) {
+  /* applyForces_LangevinIntegration, md.cpp:923-1059 */
+  const double t = dt;
+  const double gammat = gamma * t;
+  if (gammat < 0.00001) return pdo_vv_step(n, mass, dt, f, pos, vel);
+  double c[6];
+  pdo_langevin_coeffs(gammat, c);
+  const double c0 = c[0], c1 = c[1], c2 = c[2], varr = c[3], varv = c[4], crv = c[5];
+  if (step > 0) {
+    for (int i = 0; i < n; i++) {
+      double tinvmass = t / mass[i];
+      for (int d = 0; d < 3; d++)
+        vel[3 * i + d] += (1.0 - c0 / c1) * f[3 * i + d] * tinvmass / gammat;
+    }
+  }
+  double ekin = pdo_kinetic_energy(n, mass, vel);
+  if (step < (steps - 1)) {
+    const double ia = 1 / Angstrom;
+    for (int i = 0; i < n; i++) {
+      double tinvmass = t / mass[i];
+      double kTdivm = kB * temp / mass[i];
+      double sigmav = sqrt(kTdivm * varv);
+      double sigmar = sqrt(kTdivm * varr) / gamma;
+      const double *z = noise + 6 * i;
+      double n1x = z[0], n2y = z[1], n2x = z[2], n1z = z[3], n1y = z[4], n2z = z[5];
+      double s = sqrt(1.0 - sqr(crv));
+      double dvx = sigmav * (crv * n1x + s * n2x);
+      double dvy = sigmav * (crv * n1y + s * n2y);
+      double dvz = sigmav * (crv * n1z + s * n2z);
+      double drx = sigmar * n1x, dry = sigmar * n1y, drz = sigmar * n1z;
+      double rx = t * (c1 * vel[3 * i + 0] + c2 * f[3 * i + 0] * tinvmass) + drx;
+      double ry = t * (c1 * vel[3 * i + 1] + c2 * f[3 * i + 1] * tinvmass) + dry;
+      double rz = t * (c1 * vel[3 * i + 2] + c2 * f[3 * i + 2] * tinvmass) + drz;
+      rx *= ia;
+      ry *= ia;
+      rz *= ia;
+      double ux = (c0 * c2) * f[3 * i + 0] * tinvmass / c1 + dvx;
+      double uy = (c0 * c2) * f[3 * i + 1] * tinvmass / c1 + dvy;
+      double uz = (c0 * c2) * f[3 * i + 2] * tinvmass / c1 + dvz;
+      pos[3 * i + 0] += rx;
+      pos[3 * i + 1] += ry;
+      pos[3 * i + 2] += rz;
+      vel[3 * i + 0] *= c0;
+      vel[3 * i + 1] *= c0;
+      vel[3 * i + 2] *= c0;
+      vel[3 * i + 0] += ux;
+      vel[3 * i + 1] += uy;
+      vel[3 * i + 2] += uz;
+    }
+  }
+  return ekin;
+}
+
+int pdo_remd_accept(double epot_lo, double epot_hi, double T_lo, double T_hi, double uniform) {
+  double delta = (epot_lo - epot_hi) * (1.0 / (kB * T_hi) - 1.0 / (kB * T_lo));
+  return (delta <= 0) || (exp(-delta) > uniform);
+}
+
+/* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ------------------------- */
+void pdo_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+void pdo_box_muller(double u1, double u2, double *n1, double *n2) {
+  const double TwoPI = 2.0 * PD_PI;
+  double l = sqrt(-2.0 * log(u1));
+  *n1 = l * cos((-TwoPI) * u2);
+  *n2 = l * sin((-TwoPI) * u2);
+}
+
+static double u53(uint32_t a, uint32_t b) {
+  /* (0,1]: 53 random bits from two words, plus one ulp so that log() is finite */
+  uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t atom, double out[6]) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int k = 0; k < 3; k++) {
+    uint32_t ctr[4] = {atom, step, stream, (uint32_t)k};
+    uint32_t w[4];
+    pdo_philox4x32(ctr, key, w);
+    pdo_box_muller(u53(w[0], w[1]), u53(w[2], w[3]), &out[2 * k], &out[2 * k + 1]);
+  }
+}

@@ -1,0 +1,197 @@
+/*
+ * pd_oracle.h -- FP64 CPU restatement of PD's nonbonded / GB-SA path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP engine
+ * is compared against.  It may be loaded only by tests/, by the smoke check in
+ * __graft_entry__.py and by the cpu_baseline leg of bench.py.  The product
+ * library (pd_amd/libpdgpu.so) never links or calls it.
+ *
+ * Every function restates the PD function named in its comment, in PD's
+ * arithmetic order (left-to-right sums, no FMA contraction: the file is built
+ * with -ffp-contract=off), so that the pair set and the FP64 energies agree
+ * with PD's CPU code on the same inputs.
+ *
+ * Units follow PD: positions in Angstrom, forces in J/m, energies in J per
+ * molecule, masses in kg, velocities in m/s, per-atom epsilon stored as
+ * sqrt(epsilon[J]) (ffparam.cpp:1216), radius = R_min/2 in Angstrom.
+ */
+#ifndef PD_ORACLE_H
+#define PD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Energies, mirroring the members of PD's Hamiltonian (hamiltonian.h:40-66). */
+typedef struct pdo_energies {
+  double epot;
+  double epot_vdw;
+  double epot_elec;
+  double epot_pol;
+  double epot_pol_self;
+  double epot_pol_cross;
+  double epot_bond;
+  double epot_angle;
+  double epot_torsion;
+  double epot_surf;
+  double virial;       /* InternalVirial */
+  double ekin;
+} pdo_energies;
+
+/* FF_NonBonded public parameters (nonbonded.h:72-143, defaults nonbonded.cpp:985-1021). */
+typedef struct pdo_nb_params {
+  double Cutoff, InnerCutoff, VdwCutoff, VdwInnerCutoff;
+  double Vdw14Scaling, Elec14Scaling;
+  double Dielectric, DDDielectricAlpha;
+  int DoVdw, DoElec, ForceSwitch, EnergySwitch;
+  int VdwCor;
+  double VdwCorDensity, VdwCorEpsilon, VdwCorRadius, VdwCorCutoff;
+} pdo_nb_params;
+
+/* FF_GeneralizedBorn_Still parameters (gbff.cpp:34-55) plus the inherited
+ * nonbonded cutoffs it uses in FastMode. */
+typedef struct pdo_gb_params {
+  double Cutoff, InnerCutoff, VdwCutoff, VdwInnerCutoff;
+  double Vdw14Scaling, Elec14Scaling;
+  double DielectricSolute, DielectricSolvent;
+  double GbsaStillCutoff, ExpApproxThreshold, DielectricOffset, BornRadiusOffset;
+  double P1, P2, P3, P4, P5;
+} pdo_gb_params;
+
+/* Physical constants of forcefield.cpp:35-55, in this order:
+ * kcal2J J2kcal Na kcal2JDivNa kB R_kcal amu Angstrom invAngstrom e_charge
+ * e0 _4pi_e0 econv halfeconv econv_joule halfeconv_joule */
+void pdo_constants(double out[16]);
+
+/* PeriodicBox::setupCellDimensions basis vectors (space.cpp:47-110) as copied by
+ * FF_NonBonded::setupBasisVectors (nonbonded.cpp:1055-1074).  periodic=0 gives
+ * InfiniteSpace's single zero vector (space.h:60-61).  Returns the number of
+ * vectors, or -1 when PD would throw (more than 32 vectors). */
+int pdo_basis_vectors(int periodic, const double box[3], double cutoff, double basis[32][3]);
+
+/* WorkSpace::cleanSpace (workspace.cpp:371-380): each molecule's centre of
+ * geometry (molecule.cpp:850-858) is moved into the box by
+ * PeriodicBox::getClosestImage (space.h:215-227).  pos is AoS xyz. */
+void pdo_clean_space(const double box[3], int n_mol, const int *mol_first, const int *mol_last,
+                     double *pos);
+
+/* BondOrder::calcMaxIndexDelta (bondorder.cpp:94-150).  cov lists are CSR
+ * (off[n+1], idx[]); resid[] is each atom's residue index. */
+int pdo_bondorder_mid(int n, const int *resid,
+                      const int *off12, const int *idx12,
+                      const int *off13, const int *idx13,
+                      const int *off14, const int *idx14);
+
+/* BondOrder::reinit/calcOrder (bondorder.cpp:170-212).  band has mid*(n-1)
+ * entries.  Sparse (off-diagonal) partners are written in PD's insertion order
+ * to sp_i/sp_j/sp_order (capacity sp_cap); returns their number or -1 on
+ * overflow. */
+int pdo_bondorder_fill(int n, int mid, int max_bond_order,
+                       const int *off12, const int *idx12,
+                       const int *off13, const int *idx13,
+                       const int *off14, const int *idx14,
+                       signed char *band, int *sp_i, int *sp_j, int *sp_order, int sp_cap);
+
+/* NeighbourList::calcNewList (neighbourlist.cpp:222-244) with
+ * calcNewList_PeriodicBox (336-439) or calcNewList_InfiniteSpace (247-333),
+ * then correctOffDiagonalBondOrders (179-220).  Writes per-atom counts and the
+ * packed entries j | bo<<24 | image<<27 atom after atom into packed (capacity
+ * cap).  Returns the total count, or -(needed) when cap is too small. */
+int64_t pdo_build_list(int n, const double *pos, int periodic, const double box[3],
+                       double cutoff, double padding, int calc_shadow,
+                       int mid, const signed char *band,
+                       int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
+                       int *counts, uint32_t *packed, int64_t cap);
+
+/* FF_NonBonded::calcForces -> FF_NonBonded_CalcForces_T_fast2 (nonbonded.cpp:559-910,
+ * dispatch 1222-1246).  Adds forces into f (AoS, J/m).  Energies follow PD: epot_vdw
+ * and epot_elec are assigned, epot and virial are added to. */
+void pdo_nb_forces(int n, const double *pos, const double *radius, const double *epsilon,
+                   const double *charge, const int *counts, const uint32_t *packed,
+                   const double basis[32][3], const pdo_nb_params *p, double *f,
+                   pdo_energies *e);
+
+/* Sum over the pairs pdo_nb_forces evaluates of |E_vdw| + |E_elec| (J): the
+ * condition scale of the total energy, used to bound summation error. */
+double pdo_nb_abs_energy(int n, const double *pos, const double *radius, const double *epsilon,
+                         const double *charge, const int *counts, const uint32_t *packed,
+                         const double basis[32][3], const pdo_nb_params *p);
+
+/* FF_GeneralizedBorn_Still::calcFixedBornRadiiTerms (gbff.cpp:324-398).
+ * gbrad[i] is the GB radius of atom i's type, cov12/cov13 lists carry the
+ * equilibrium distances d (fundamentals.h:80-90). */
+void pdo_gb_fixed_terms(int n, const double *gbrad, const pdo_gb_params *p,
+                        const int *off12, const int *idx12, const double *d12,
+                        const int *off13, const int *idx13, const double *d13,
+                        double *Vj, double *terms123, double *bornradius0);
+
+/* calcBornRadii_PairwiseApprox (gbff.cpp:432-483) over a full (shadow) list. */
+void pdo_gb_radii(int n, const double *pos, const double *gbrad, const double *Vj,
+                  const double *terms123, const pdo_gb_params *p, const int *counts,
+                  const uint32_t *packed, double *bornradius);
+
+/* calcForcesIncludingVacuo (gbff.cpp:1466-1835): GB + vacuum Coulomb + LJ pair
+ * pass, self terms and the Born-radius chain rule.  Adds forces to f (J/m). */
+void pdo_gb_forces_fast(int n, const double *pos, const double *radius, const double *epsilon,
+                        const double *charge, const double *gbrad, const double *Vj,
+                        const double *bornradius, const pdo_gb_params *p, const int *counts,
+                        const uint32_t *packed, double *f, pdo_energies *e, double *atom_epot);
+
+/* Energy-only GB path FF_GeneralizedBorn_Still::calcEnergies (gbff.cpp:178-183):
+ * fast2 nonbonded energies with the GB's vacuum settings, then calcBornEnergy
+ * (gbff.cpp:537-596).  bornradius must come from pdo_gb_radii. */
+void pdo_gb_energies(int n, const double *pos, const double *radius, const double *epsilon,
+                     const double *charge, const double *bornradius, const pdo_gb_params *p,
+                     const int *counts, const uint32_t *packed, pdo_energies *e);
+
+/* FF_Bonded bond / angle / torsion forces (ffbonded.cpp:945-992, 1054-1177, 1274-1512).
+ * Force constants in PD units (J/A^2, J/rad^2, J).  Adds to f and e. */
+void pdo_bond_forces(const double *pos, int nb, const int *bi, const int *bj,
+                     const double *bl, const double *bk, double *f, pdo_energies *e);
+void pdo_angle_forces(const double *pos, int na, const int *ai, const int *aa, const int *aj,
+                      const double *theta0, const double *ak, double *f, pdo_energies *e);
+/* Torsions with up to 4 Fourier terms each: Vn[4*t+k], n[4*t+k], gamma[4*t+k], terms[t]. */
+void pdo_torsion_forces(const double *pos, int nt, const int *ti, const int *ta, const int *tb,
+                        const int *tj, const int *terms, const double *Vn, const double *nn,
+                        const double *gamma, double *f, pdo_energies *e);
+
+/* MolecularDynamics::calcKineticEnergy (md.cpp:521-556). */
+double pdo_kinetic_energy(int n, const double *mass, const double *vel);
+
+/* applyForces_VelocityVerletIntegration (md.cpp:805-855), Thermostat/Barostat
+ * off.  Returns the kinetic energy computed between the two loops. */
+double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
+                   double *vel);
+
+/* applyForces_LangevinIntegration (md.cpp:923-1059) for step `step` of `steps`,
+ * with the six unit normal deviates per atom supplied in `noise` (n1x n2y n2x n1z
+ * n1y n2z, the order of the three nrand calls).  Returns the kinetic energy. */
+double pdo_langevin_step(int n, const double *mass, double dt, double gamma, double temp,
+                         int step, int steps, const double *f, const double *noise,
+                         double *pos, double *vel);
+
+/* Langevin coefficients c0 c1 c2 varr varv crv (md.cpp:947-986). */
+void pdo_langevin_coeffs(double gammat, double out[6]);
+
+/* REX_Local::exchangeCriterion (remd.cpp:349-357) with the uniform deviate
+ * supplied. */
+int pdo_remd_accept(double epot_lo, double epot_hi, double T_lo, double T_hi, double uniform);
+
+/* Philox4x32-10 counter-based generator (Salmon et al., SC'11), the stream the
+ * device integrator draws from: returns 4 words for (counter, key). */
+void pdo_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Unit normal pair from two uniforms in (0,1], in the form of PD's nrand
+ * (maths.h:155-169). */
+void pdo_box_muller(double u1, double u2, double *n1, double *n2);
+
+/* Langevin noise for atom `atom` at step `step` from (seed): six unit normals
+ * in the order pdo_langevin_step consumes them. */
+void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t atom, double out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

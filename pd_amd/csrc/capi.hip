@@ -1,0 +1,427 @@
+// capi.hip -- the extern "C" entry points of include/pdgpu.h.
+//
+// Each entry point catches every exception, records its text on the handle and
+// returns a negative code, so no C++ exception crosses the ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "engine.hpp"
+
+namespace pdgpu {
+
+Engine::Engine(int natoms, int dev) : n(natoms), device(dev) {
+  PDGPU_HIP(hipSetDevice(dev));
+  PDGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // InfiniteSpace, an empty band (every pair order 7) until set.
+  mid = 1;
+  h_band.assign(std::max(n - 1, 1), (int8_t)7);
+  band.upload(h_band, stream);
+  std::vector<int> off(n + 1, 0);
+  sp_off.upload(off, stream);
+  sp_part.resize(1);
+  sp_ord.resize(1);
+}
+
+Engine::~Engine() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+void upload_sparse(Engine &E) {
+  // per-atom CSR of off-diagonal partners keyed by the lower index, in PD order
+  std::vector<int> cnt(E.n + 1, 0);
+  for (size_t k = 0; k < E.h_sp_i.size(); k++) cnt[std::min(E.h_sp_i[k], E.h_sp_j[k]) + 1]++;
+  for (int i = 0; i < E.n; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> part(std::max<size_t>(E.h_sp_i.size(), 1)), ord(std::max<size_t>(E.h_sp_i.size(), 1));
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (size_t k = 0; k < E.h_sp_i.size(); k++) {
+    int lo = std::min(E.h_sp_i[k], E.h_sp_j[k]), hi = std::max(E.h_sp_i[k], E.h_sp_j[k]);
+    part[fill[lo]] = hi;
+    ord[fill[lo]] = E.h_sp_order[k];
+    fill[lo]++;
+  }
+  E.sp_off.upload(cnt, E.stream);
+  E.sp_part.upload(part, E.stream);
+  E.sp_ord.upload(ord, E.stream);
+}
+
+void read_aos(const double *src, size_t stride, int n, std::vector<double> &a, std::vector<double> &b,
+              std::vector<double> &c) {
+  a.resize(n);
+  b.resize(n);
+  c.resize(n);
+  for (int i = 0; i < n; i++) {
+    const double *p = reinterpret_cast<const double *>(reinterpret_cast<const char *>(src) + (size_t)i * stride);
+    a[i] = p[0];
+    b[i] = p[1];
+    c[i] = p[2];
+  }
+}
+
+void write_aos(double *dst, size_t stride, int n, const std::vector<double> &a, const std::vector<double> &b,
+               const std::vector<double> &c) {
+  for (int i = 0; i < n; i++) {
+    double *p = reinterpret_cast<double *>(reinterpret_cast<char *>(dst) + (size_t)i * stride);
+    p[0] = a[i];
+    p[1] = b[i];
+    p[2] = c[i];
+  }
+}
+
+template <typename F>
+int guarded(pdgpu_engine *h, F &&fn) {
+  Engine *e = reinterpret_cast<Engine *>(h);
+  if (!e) return PDGPU_ERR_ARG;
+  try {
+    PDGPU_HIP(hipSetDevice(e->device));
+    fn(*e);
+    return PDGPU_OK;
+  } catch (const EngineError &x) {
+    e->last_error = x.what();
+    return x.code;
+  } catch (const std::bad_alloc &) {
+    e->last_error = "host allocation failed";
+    return PDGPU_ERR_NOMEM;
+  } catch (const std::exception &x) {
+    e->last_error = x.what();
+    return PDGPU_ERR_ARG;
+  }
+}
+
+}  // namespace
+}  // namespace pdgpu
+
+using pdgpu::Engine;
+using pdgpu::EngineError;
+
+extern "C" {
+
+const char *pdgpu_version(void) {
+  return "pdgpu 0.1 gfx950";
+}
+
+int pdgpu_create(int32_t n_atoms, int32_t device, pdgpu_engine **out) {
+  if (!out || n_atoms <= 0 || n_atoms > (1 << 24) - 1) return PDGPU_ERR_ARG;
+  try {
+    *out = reinterpret_cast<pdgpu_engine *>(new Engine(n_atoms, device));
+    return PDGPU_OK;
+  } catch (const EngineError &x) {
+    *out = nullptr;
+    return x.code;
+  } catch (...) {
+    *out = nullptr;
+    return PDGPU_ERR_HIP;
+  }
+}
+
+int pdgpu_destroy(pdgpu_engine *h) {
+  Engine *e = reinterpret_cast<Engine *>(h);
+  if (!e) return PDGPU_ERR_ARG;
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  delete e;
+  return PDGPU_OK;
+}
+
+const char *pdgpu_last_error(const pdgpu_engine *h) {
+  const Engine *e = reinterpret_cast<const Engine *>(h);
+  return e ? e->last_error.c_str() : "null engine handle";
+}
+
+int pdgpu_set_atom_params(pdgpu_engine *h, const double *radius, const double *epsilon, const double *charge,
+                          const double *mass) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(radius && epsilon && charge && mass, "null parameter array");
+    E.h_radius.assign(radius, radius + E.n);
+    E.h_epsilon.assign(epsilon, epsilon + E.n);
+    E.h_charge.assign(charge, charge + E.n);
+    E.h_mass.assign(mass, mass + E.n);
+    for (int i = 0; i < E.n; i++) E.require(mass[i] > 0.0, "masses must be positive");
+    E.mass.upload(E.h_mass, E.stream);
+    if (E.have_nb) E.setup_nonbonded(E.nb);
+  });
+}
+
+int pdgpu_set_molecules(pdgpu_engine *h, int32_t n_mol, const int32_t *first, const int32_t *last) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(n_mol >= 0 && (n_mol == 0 || (first && last)), "bad molecule arrays");
+    for (int m = 0; m < n_mol; m++)
+      E.require(first[m] >= 0 && last[m] >= first[m] && last[m] < E.n, "molecule range out of bounds");
+    E.h_mol_first.assign(first, first + n_mol);
+    E.h_mol_last.assign(last, last + n_mol);
+    E.mol_first.upload(E.h_mol_first, E.stream);
+    E.mol_last.upload(E.h_mol_last, E.stream);
+  });
+}
+
+int pdgpu_set_bond_order(pdgpu_engine *h, int32_t max_index_delta, const int8_t *band, int32_t n_sparse,
+                         const int32_t *sp_i, const int32_t *sp_j, const int32_t *sp_order) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(max_index_delta >= 1, "max_index_delta must be >= 1");
+    E.require(band != nullptr || E.n == 1, "null band");
+    E.require(n_sparse >= 0 && (n_sparse == 0 || (sp_i && sp_j && sp_order)), "bad sparse arrays");
+    E.mid = max_index_delta;
+    size_t nb = (size_t)max_index_delta * (size_t)std::max(E.n - 1, 1);
+    E.h_band.assign(band, band + (E.n > 1 ? nb : 0));
+    if (E.h_band.empty()) E.h_band.assign(1, 7);
+    E.band.upload(E.h_band, E.stream);
+    E.h_sp_i.assign(sp_i, sp_i + n_sparse);
+    E.h_sp_j.assign(sp_j, sp_j + n_sparse);
+    E.h_sp_order.assign(sp_order, sp_order + n_sparse);
+    for (int k = 0; k < n_sparse; k++)
+      E.require(sp_i[k] >= 0 && sp_i[k] < E.n && sp_j[k] >= 0 && sp_j[k] < E.n, "sparse index out of bounds");
+    pdgpu::upload_sparse(E);
+    E.list_valid = false;
+  });
+}
+
+int pdgpu_set_bond_order_from_cov(pdgpu_engine *h, const int32_t *resid, const int32_t *off12,
+                                  const int32_t *idx12, const int32_t *off13, const int32_t *idx13,
+                                  const int32_t *off14, const int32_t *idx14) {
+  Engine *e = reinterpret_cast<Engine *>(h);
+  if (!e || !resid || !off12 || !off13 || !off14) return PDGPU_ERR_ARG;
+  const int n = e->n;
+  // BondOrder::calcMaxIndexDelta (bondorder.cpp:94-150)
+  int mid = 0;
+  const int32_t *offs[3] = {off12, off13, off14};
+  const int32_t *idxs[3] = {idx12, idx13, idx14};
+  for (int i = 0; i < n; i++)
+    for (int l = 0; l < 3; l++)
+      for (int k = offs[l][i]; k < offs[l][i + 1]; k++) {
+        int o = idxs[l][k];
+        if (o < 0 || o >= n) return PDGPU_ERR_ARG;
+        if (std::abs(resid[i] - resid[o]) > 1) continue;
+        mid = std::max(mid, std::abs(i - o));
+      }
+  mid += 1;
+  // BondOrder::reinit / calcOrder (bondorder.cpp:170-212): 1-4, then 1-3, then 1-2
+  std::vector<int8_t> band((size_t)mid * std::max(n - 1, 1), (int8_t)7);
+  std::vector<int> si, sj, so;
+  const int32_t *o2[3] = {off14, off13, off12};
+  const int32_t *i2[3] = {idx14, idx13, idx12};
+  const int order[3] = {3, 2, 1};
+  for (int i = 0; i < n; i++)
+    for (int l = 0; l < 3; l++)
+      for (int k = o2[l][i]; k < o2[l][i + 1]; k++) {
+        int j = i2[l][k];
+        if (j <= i) continue;
+        int delta = j - i;
+        int ord = std::min(order[l], 7);
+        if (delta > mid) {
+          si.push_back(i);
+          sj.push_back(j);
+          so.push_back(ord);
+        } else {
+          band[(size_t)i * mid + (delta - 1)] = (int8_t)ord;
+        }
+      }
+  return pdgpu_set_bond_order(h, mid, band.data(), (int32_t)si.size(), si.data(), sj.data(), so.data());
+}
+
+int pdgpu_set_space(pdgpu_engine *h, int32_t periodic, const double *box) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.periodic = periodic != 0;
+    if (E.periodic) {
+      E.require(box && box[0] > 0 && box[1] > 0 && box[2] > 0, "box lengths must be positive");
+      for (int d = 0; d < 3; d++) E.box[d] = box[d];
+    } else {
+      for (int d = 0; d < 3; d++) E.box[d] = 0.0;
+    }
+    E.list_valid = false;
+  });
+}
+
+int pdgpu_request_cutoff(pdgpu_engine *h, double cutoff) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    if (cutoff > E.list_cutoff) E.list_cutoff = cutoff;
+  });
+}
+
+int pdgpu_set_padding(pdgpu_engine *h, double padding) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(padding >= 0.0, "Neighborlist Padding cannot be < 0");
+    E.padding = padding;
+  });
+}
+
+int pdgpu_calc_shadow(pdgpu_engine *h, int32_t enabled) {
+  return pdgpu::guarded(h, [&](Engine &E) { E.calc_shadow = enabled != 0; });
+}
+
+int pdgpu_setup_nonbonded(pdgpu_engine *h, const pdgpu_nb_params *p) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(p != nullptr, "null parameters");
+    E.require(p->Cutoff > 0 && p->Dielectric != 0.0, "bad nonbonded parameters");
+    E.setup_nonbonded(*p);
+  });
+}
+
+int pdgpu_set_bonded(pdgpu_engine *h, int32_t nb, const int32_t *bi, const int32_t *bj, const double *bl,
+                     const double *bk, int32_t na, const int32_t *ai, const int32_t *aa, const int32_t *aj,
+                     const double *theta0, const double *ak, int32_t nt, const int32_t *ti, const int32_t *ta,
+                     const int32_t *tb, const int32_t *tj, const int32_t *terms, const double *vn,
+                     const double *nn, const double *gamma) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(nb >= 0 && na >= 0 && nt >= 0, "negative term count");
+    auto chk = [&](const int32_t *a, int c) {
+      for (int k = 0; k < c; k++) E.require(a[k] >= 0 && a[k] < E.n, "bonded atom index out of bounds");
+    };
+    if (nb) { chk(bi, nb); chk(bj, nb); }
+    if (na) { chk(ai, na); chk(aa, na); chk(aj, na); }
+    if (nt) { chk(ti, nt); chk(ta, nt); chk(tb, nt); chk(tj, nt); }
+    const double J2kcal = 1.0 / pdgpu::Consts::kcal_to_J();  // J per molecule -> kcal/mol
+    auto ivec = [](const int32_t *a, int c) { return std::vector<int>(a, a + c); };
+    auto dvec = [](const double *a, int c, double s) {
+      std::vector<double> v(c);
+      for (int k = 0; k < c; k++) v[k] = a[k] * s;
+      return v;
+    };
+    E.nbonds = nb;
+    E.nangles = na;
+    E.ntors = nt;
+    if (nb) {
+      E.b_i.upload(ivec(bi, nb), E.stream);
+      E.b_j.upload(ivec(bj, nb), E.stream);
+      E.b_l.upload(dvec(bl, nb, 1.0), E.stream);
+      E.b_k.upload(dvec(bk, nb, J2kcal), E.stream);
+    }
+    if (na) {
+      E.a_i.upload(ivec(ai, na), E.stream);
+      E.a_a.upload(ivec(aa, na), E.stream);
+      E.a_j.upload(ivec(aj, na), E.stream);
+      E.a_t0.upload(dvec(theta0, na, 1.0), E.stream);
+      E.a_k.upload(dvec(ak, na, J2kcal), E.stream);
+    }
+    if (nt) {
+      E.t_i.upload(ivec(ti, nt), E.stream);
+      E.t_a.upload(ivec(ta, nt), E.stream);
+      E.t_b.upload(ivec(tb, nt), E.stream);
+      E.t_j.upload(ivec(tj, nt), E.stream);
+      E.t_terms.upload(ivec(terms, nt), E.stream);
+      E.t_vn.upload(dvec(vn, 4 * nt, J2kcal), E.stream);
+      E.t_nn.upload(dvec(nn, 4 * nt, 1.0), E.stream);
+      E.t_gamma.upload(dvec(gamma, 4 * nt, 1.0), E.stream);
+    }
+    E.sync();
+  });
+}
+
+int pdgpu_set_positions(pdgpu_engine *h, const double *xyz, size_t stride_bytes) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(xyz && stride_bytes >= 3 * sizeof(double), "bad position array");
+    std::vector<double> a, b, c;
+    pdgpu::read_aos(xyz, stride_bytes, E.n, a, b, c);
+    E.x.upload(a, E.stream);
+    E.y.upload(b, E.stream);
+    E.z.upload(c, E.stream);
+    if (E.vx.n < (size_t)E.n) {
+      std::vector<double> zero(E.n, 0.0);
+      E.vx.upload(zero, E.stream);
+      E.vy.upload(zero, E.stream);
+      E.vz.upload(zero, E.stream);
+    }
+    E.sync();
+    E.have_positions = true;
+  });
+}
+
+int pdgpu_get_positions(pdgpu_engine *h, double *xyz, size_t stride_bytes) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(E.have_positions && xyz && stride_bytes >= 3 * sizeof(double), "bad call");
+    std::vector<double> a(E.n), b(E.n), c(E.n);
+    E.x.download(a.data(), E.n, E.stream);
+    E.y.download(b.data(), E.n, E.stream);
+    E.z.download(c.data(), E.n, E.stream);
+    E.sync();
+    pdgpu::write_aos(xyz, stride_bytes, E.n, a, b, c);
+  });
+}
+
+int pdgpu_set_velocities(pdgpu_engine *h, const double *xyz, size_t stride_bytes) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(xyz && stride_bytes >= 3 * sizeof(double), "bad velocity array");
+    std::vector<double> a, b, c;
+    pdgpu::read_aos(xyz, stride_bytes, E.n, a, b, c);
+    E.vx.upload(a, E.stream);
+    E.vy.upload(b, E.stream);
+    E.vz.upload(c, E.stream);
+    E.sync();
+  });
+}
+
+int pdgpu_get_velocities(pdgpu_engine *h, double *xyz, size_t stride_bytes) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(xyz && stride_bytes >= 3 * sizeof(double) && E.vx.n >= (size_t)E.n, "bad call");
+    std::vector<double> a(E.n), b(E.n), c(E.n);
+    E.vx.download(a.data(), E.n, E.stream);
+    E.vy.download(b.data(), E.n, E.stream);
+    E.vz.download(c.data(), E.n, E.stream);
+    E.sync();
+    pdgpu::write_aos(xyz, stride_bytes, E.n, a, b, c);
+  });
+}
+
+int pdgpu_build_list(pdgpu_engine *h) {
+  return pdgpu::guarded(h, [&](Engine &E) { E.build_list(); });
+}
+
+int pdgpu_full_update_count(pdgpu_engine *h, uint32_t *count) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(count != nullptr, "null count");
+    *count = E.full_update_count;
+  });
+}
+
+int pdgpu_get_pair_count(pdgpu_engine *h, int64_t *total) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(total != nullptr, "null total");
+    std::vector<int> counts;
+    std::vector<uint32_t> packed;
+    E.export_pairs(counts, packed);
+    *total = (int64_t)packed.size();
+  });
+}
+
+int pdgpu_get_pairs(pdgpu_engine *h, int32_t *counts, uint32_t *packed, int64_t capacity) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(counts && packed, "null output arrays");
+    std::vector<int> c;
+    std::vector<uint32_t> p;
+    E.export_pairs(c, p);
+    E.require((int64_t)p.size() <= capacity, "capacity too small for the pair set");
+    std::memcpy(counts, c.data(), sizeof(int) * c.size());
+    std::memcpy(packed, p.data(), sizeof(uint32_t) * p.size());
+  });
+}
+
+int pdgpu_compute(pdgpu_engine *h, uint32_t components, double *f_accum, size_t stride_bytes,
+                  pdgpu_energies *energies) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(E.have_positions, "positions not set");
+    E.require(components != 0, "no component selected");
+    E.require(!f_accum || stride_bytes >= 3 * sizeof(double), "bad force stride");
+    E.evaluate(components);
+    if (f_accum) E.forces_to_host(f_accum, stride_bytes);
+    pdgpu_energies tmp;
+    E.energies_to_host(components, &tmp);
+    if (energies) *energies = tmp;
+    if (std::isnan(tmp.epot))
+      throw EngineError(PDGPU_ERR_UNSTABLE, "Simulation unstable or division by zero");
+  });
+}
+
+int pdgpu_md(pdgpu_engine *h, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
+             double *ekin_trace) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(p != nullptr && n_steps >= 0, "bad md arguments");
+    E.require(p->integrator == PDGPU_VELOCITY_VERLET || p->integrator == PDGPU_LANGEVIN, "unknown integrator");
+    E.require(p->components != 0, "no component selected");
+    if (n_steps == 0) return;
+    E.md(*p, n_steps, epot_trace, ekin_trace);
+  });
+}
+
+}  // extern "C"

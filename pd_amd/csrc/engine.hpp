@@ -1,0 +1,240 @@
+// engine.hpp -- internal state of one pdgpu engine (one molecular system in HBM).
+//
+// Layout in HBM (all device buffers are owned by Engine):
+//   original order (PD atom index):  x,y,z,vx,vy,vz (f64 SoA), mass (f64),
+//       bonded+GB force accumulators fbx,fby,fbz (int64 fixed point, 2^32 per
+//       kcal/mol/A), molecules (first,last), bond-order band/sparse tables.
+//   cluster order (rebuilt with the list): slots of 8 atoms per cluster, atoms
+//       of a cluster taken from one grid column sorted by z.  slot_orig[slot]
+//       maps back to the PD index (-1 for padding); sx,sy,sz (f64) hold the
+//       positions gathered each step; sq,sr,se (f32) the scaled charge, LJ
+//       radius and sqrt(eps) per slot; nbx,nby,nbz (int64) the nonbonded force
+//       accumulators in slot order.
+//   list: for every i-cluster a contiguous run of 8-byte entries
+//       {j slot | image<<24 ; listed-mask(8) | bond-order codes(16)} (see
+//       k_list.hip), located by cl_entry_start / cl_entry_count.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pdgpu.h"
+
+namespace pdgpu {
+
+// Fixed-point scale of all force / energy-derivative accumulators: 2^32 units
+// per kcal/mol/A.  |F| up to 2^31 kcal/mol/A fits an int64.
+constexpr double kFixScale = 4294967296.0;
+constexpr float kFixScaleF = 4294967296.0f;
+constexpr int kClusterSize = 8;
+constexpr int kWave = 64;
+
+// PD constants (forcefield.cpp:35-55), FP64.
+struct Consts {
+  static constexpr double PI = 3.14159265358979323846;
+  static constexpr double kcal2J = 4184.0;
+  static constexpr double Na = 6.0221418E23;
+  static constexpr double kB = 1.38065E-23;
+  static constexpr double Angstrom = 1E-10;
+  static constexpr double e_charge = 1.60217645E-19;
+  static constexpr double e0 = 8.854187817E-12;
+  static double J2kcal() { return 1.0 / kcal2J; }
+  static double four_pi_e0() { return (PI * 4.0) * e0; }
+  static double econv() {  // kcal/mol * A
+    return (J2kcal() * Na * (e_charge * e_charge)) / (four_pi_e0() * Angstrom);
+  }
+  static double halfeconv() { return econv() * 0.5; }
+  static double econv_joule() { return (e_charge * e_charge) / (four_pi_e0() * Angstrom); }
+  static double halfeconv_joule() { return econv_joule() * 0.5; }
+  // kcal/mol -> J per molecule; kcal/mol/A -> J/m
+  static double kcal_to_J() { return kcal2J / Na; }
+  static double kcalA_to_Jm() { return kcal2J / Na / Angstrom; }
+};
+
+struct EngineError : std::runtime_error {
+  int code;
+  EngineError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define PDGPU_HIP(call)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (call);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw ::pdgpu::EngineError(PDGPU_ERR_HIP, std::string(#call) + ": " +              \
+                                                    hipGetErrorString(_e));              \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void resize(size_t count) {  // contents are not preserved
+    if (count <= n && p) return;
+    release();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess)
+      throw EngineError(PDGPU_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    n = count;
+  }
+  void upload(const T *h, size_t count, hipStream_t s) {
+    resize(count);
+    if (count) PDGPU_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  void download(T *h, size_t count, hipStream_t s) const {
+    if (count) PDGPU_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Parameters the nonbonded kernel needs, in the kernel's units (kcal/mol, A).
+struct NBKernelParams {
+  double cut2;        // Cutoff^2 (FP64: PD's per-pair test, nonbonded.cpp:720)
+  double vdwcut2;     // VdwCutoff^2
+  float cut, inner, vdwcut, vdwinner;
+  float invw2_vdw4;   // 4 / (VdwCutoff-VdwInnerCutoff)^2
+  float invw_vdw;
+  float invw_el, invw2_el4;
+  float sA, sB, sC, sD, sE, fcut, eshift;  // force-switch constants (nonbonded.cpp:620-636)
+  float rc2, rin2;
+  float vdw14, elec14;
+  int vdw_on;
+  int elec_mode;      // 0 none, 1 plain, 3 energy switch, 4 force switch
+  double vdwcor_per_atom;  // kcal/mol added per atom when VdwCor
+};
+
+struct Engine {
+  int n = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  // ---- system description -------------------------------------------------
+  std::vector<double> h_radius, h_epsilon, h_charge, h_mass;
+  std::vector<int> h_mol_first, h_mol_last;
+  int mid = 1;                 // BondOrder::m_MaxIndexDelta
+  std::vector<int8_t> h_band;  // mid*(n-1)
+  std::vector<int> h_sp_i, h_sp_j, h_sp_order;
+  bool periodic = false;
+  double box[3] = {0, 0, 0};
+  double list_cutoff = 0.0;    // NeighbourListBase::Cutoff
+  double padding = 0.5;        // NeighbourListBase::Padding default (neighbourlist.cpp:35)
+  bool calc_shadow = false;
+  uint32_t full_update_count = 0;
+  bool have_nb = false;
+  pdgpu_nb_params nb{};
+  NBKernelParams nbk{};
+  bool have_positions = false;
+  bool list_valid = false;
+
+  // ---- device state, original order -----------------------------------------
+  DBuf<double> x, y, z, vx, vy, vz, mass;
+  DBuf<long long> fbx, fby, fbz;           // bonded / GB / integrator force accumulators
+  DBuf<int> mol_first, mol_last;
+  DBuf<int8_t> band;
+  DBuf<int> sp_off, sp_part, sp_ord;       // per-atom CSR of off-diagonal partners (both directions)
+  DBuf<float> q_scaled, lj_r, lj_e;        // per atom, original order
+
+  // ---- cluster / list state ---------------------------------------------------
+  int ncx = 0, ncy = 0, ncol = 0;
+  double grid_lo[3] = {0, 0, 0}, grid_w[2] = {0, 0};
+  int max_clusters = 0;                    // allocation bound
+  DBuf<double> wx, wy, wz;                 // wrapped coordinates used for binning
+  DBuf<int> atom_col, col_count, col_fill, col_atom_start, col_cl_start, col_ncl;
+  DBuf<int> col_atoms;                     // atoms grouped by column (unsorted)
+  DBuf<int> slot_orig, orig_slot;
+  DBuf<int> n_clusters_d;                  // device scalar
+  DBuf<int> cl_col;                        // column of each cluster
+  DBuf<double> cl_lo, cl_hi;               // cluster bounding boxes (wrapped frame), 3 per cluster
+  DBuf<double> sx, sy, sz;                 // gathered positions, slot order
+  DBuf<float> sq, sr, se;                  // per-slot parameters
+  DBuf<long long> nbx, nby, nbz;           // nonbonded force accumulators, slot order
+  DBuf<uint2> entries;
+  DBuf<int> cl_entry_start, cl_entry_count;
+  DBuf<unsigned long long> entry_counter;  // [0] entries used, [1] overflow flag
+  size_t entry_capacity = 0;
+  int n_clusters = 0;                      // host copy after a build
+
+  // ---- reductions ---------------------------------------------------------------
+  DBuf<double> e_partial;                  // per-wave nonbonded energy partials (4 per wave)
+  DBuf<double> e_partial_bonded;           // per-block bonded energy partials (4 per block)
+  DBuf<double> e_partial_ke;               // per-block kinetic energy partials (4 per block)
+  DBuf<double> e_out;                      // reduced energies, 4 per component (see Slots)
+  DBuf<double> trace;                      // per-step epot / ekin of pdgpu_md
+
+  // ---- bonded terms -------------------------------------------------------------
+  int nbonds = 0, nangles = 0, ntors = 0;
+  DBuf<int> b_i, b_j, a_i, a_a, a_j, t_i, t_a, t_b, t_j, t_terms;
+  DBuf<double> b_l, b_k, a_t0, a_k, t_vn, t_nn, t_gamma;
+
+  explicit Engine(int natoms, int dev);
+  ~Engine();
+
+  void require(bool cond, const char *msg) const {
+    if (!cond) throw EngineError(PDGPU_ERR_ARG, msg);
+  }
+  void sync() { PDGPU_HIP(hipStreamSynchronize(stream)); }
+
+  // list.hip
+  void build_list();
+  void export_pairs(std::vector<int> &counts, std::vector<uint32_t> &packed);
+  // nonbonded.hip
+  void setup_nonbonded(const pdgpu_nb_params &p);
+  void gather_positions();
+  void launch_nonbonded(double *energy_slots);  // energies: vdw, elec, virial
+  // bonded.hip
+  void launch_bonded(double *energy_slots);      // energies: bond, angle, torsion, virial
+  // integrate.hip
+  void zero_forces();
+  void forces_to_host(double *f, size_t stride_bytes);
+  void evaluate(uint32_t components);        // forces + energies into e_out
+  void energies_to_host(uint32_t components, pdgpu_energies *out);
+  void integrate(const pdgpu_md_params &p, int step, double *ekin_slot);
+  void md(const pdgpu_md_params &p, int n_steps, double *epot_trace, double *ekin_trace);
+  uint32_t last_components = 0;
+  // k_gb.hip
+  bool have_gb = false;
+  void launch_gb(double *energy_slots);
+  void gb_energies_to_host(const double *slots, pdgpu_energies &e);
+};
+
+void langevin_coefficients(double gammat, double *c6);
+void reduce4(hipStream_t s, const double *part, int count, double *out);
+
+// ---- device helpers ------------------------------------------------------------
+
+__device__ __forceinline__ long long fix_from_float(float v) {
+  return static_cast<long long>(v * kFixScaleF);
+}
+__device__ __forceinline__ long long fix_from_double(double v) {
+  return static_cast<long long>(v * kFixScale);
+}
+__device__ __forceinline__ void fix_add(long long *p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+
+// Wave-wide sum of a double in a fixed butterfly order (deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace pdgpu

@@ -1,0 +1,200 @@
+// k_bonded.hip -- FF_Bonded bond, angle and torsion forces on the device (FP64).
+//
+// One thread per term, PD's formulas (ffbonded.cpp:945-992 bonds, 1054-1177
+// angles, 1274-1512 dihedrals) in kcal/mol and Angstrom.  Forces go to the
+// original-order 64-bit fixed-point accumulators, energies to per-block FP64
+// partials reduced in a fixed order.
+#include "engine.hpp"
+
+namespace pdgpu {
+
+namespace {
+
+__device__ __forceinline__ void add3(long long *fx, long long *fy, long long *fz, int i, double gx, double gy,
+                                     double gz) {
+  fix_add(&fx[i], fix_from_double(gx));
+  fix_add(&fy[i], fix_from_double(gy));
+  fix_add(&fz[i], fix_from_double(gz));
+}
+
+__device__ void block_partial(double v0, double v1, double v2, double v3, double *out) {
+  __shared__ double s[4][256];
+  s[0][threadIdx.x] = v0;
+  s[1][threadIdx.x] = v1;
+  s[2][threadIdx.x] = v2;
+  s[3][threadIdx.x] = v3;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 4; d++) s[d][threadIdx.x] += s[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_bonded(
+    int nb, const int *__restrict__ bi, const int *__restrict__ bj, const double *__restrict__ bl,
+    const double *__restrict__ bk, int na, const int *__restrict__ ai, const int *__restrict__ aa,
+    const int *__restrict__ aj, const double *__restrict__ at0, const double *__restrict__ ak, int nt,
+    const int *__restrict__ ti, const int *__restrict__ ta, const int *__restrict__ tb,
+    const int *__restrict__ tj, const int *__restrict__ tterms, const double *__restrict__ tvn,
+    const double *__restrict__ tnn, const double *__restrict__ tgamma, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, long long *__restrict__ fx,
+    long long *__restrict__ fy, long long *__restrict__ fz, double *__restrict__ partial) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double eb = 0.0, ea = 0.0, et = 0.0, vir = 0.0;
+  if (t < nb) {
+    const int i = bi[t], j = bj[t];
+    const double k = bk[t];
+    double vx = x[i] - x[j], vy = y[i] - y[j], vz = z[i] - z[j];
+    const double d = sqrt(vx * vx + vy * vy + vz * vz);
+    const double fm = 2 * k * (d - bl[t]);
+    vir = d * fm;
+    eb = k * (d - bl[t]) * (d - bl[t]);
+    const double s = fm / d;
+    vx *= s;
+    vy *= s;
+    vz *= s;
+    add3(fx, fy, fz, i, -vx, -vy, -vz);
+    add3(fx, fy, fz, j, vx, vy, vz);
+  } else if (t < nb + na) {
+    const int q = t - nb;
+    const double k = ak[q];
+    if (k != 0.0) {
+      const int i = ai[q], a = aa[q], j = aj[q];
+      const double ivx = x[i] - x[a], ivy = y[i] - y[a], ivz = z[i] - z[a];
+      const double inv_iv = 1 / sqrt(ivx * ivx + ivy * ivy + ivz * ivz);
+      const double uix = ivx * inv_iv, uiy = ivy * inv_iv, uiz = ivz * inv_iv;
+      const double jvx = x[j] - x[a], jvy = y[j] - y[a], jvz = z[j] - z[a];
+      const double inv_jv = 1 / sqrt(jvx * jvx + jvy * jvy + jvz * jvz);
+      const double ujx = jvx * inv_jv, ujy = jvy * inv_jv, ujz = jvz * inv_jv;
+      const double cs = uix * ujx + uiy * ujy + uiz * ujz;
+      const double th = acos(cs);
+      const double sn = sin(th);
+      ea = k * (th - at0[q]) * (th - at0[q]);
+      const double fm = 2 * k * (th - at0[q]);
+      double bx = fm * inv_iv * (uix * cs - ujx) / sn;
+      double by = fm * inv_iv * (uiy * cs - ujy) / sn;
+      double bz = fm * inv_iv * (uiz * cs - ujz) / sn;
+      add3(fx, fy, fz, i, -bx, -by, -bz);
+      double ax = bx, ay = by, az = bz;
+      bx = fm * inv_jv * (ujx * cs - uix) / sn;
+      by = fm * inv_jv * (ujy * cs - uiy) / sn;
+      bz = fm * inv_jv * (ujz * cs - uiz) / sn;
+      add3(fx, fy, fz, j, -bx, -by, -bz);
+      add3(fx, fy, fz, a, ax + bx, ay + by, az + bz);
+    }
+  } else if (t < nb + na + nt) {
+    const int q = t - nb - na;
+    const int i0 = ti[q], i1 = ta[q], i2 = tb[q], i3 = tj[q];
+    const double v1x = x[i0] - x[i1], v1y = y[i0] - y[i1], v1z = z[i0] - z[i1];
+    const double v2x = x[i1] - x[i2], v2y = y[i1] - y[i2], v2z = z[i1] - z[i2];
+    const double v3x = x[i2] - x[i3], v3y = y[i2] - y[i3], v3z = z[i2] - z[i3];
+    double n1x = v1y * v2z - v1z * v2y, n1y = v1z * v2x - v1x * v2z, n1z = v1x * v2y - v1y * v2x;
+    double n2x = v2y * v3z - v2z * v3y, n2y = v2z * v3x - v2x * v3z, n2z = v2x * v3y - v2y * v3x;
+    double n3x = v2y * n1z - v2z * n1y, n3y = v2z * n1x - v2x * n1z, n3z = v2x * n1y - v2y * n1x;
+    const double in1 = 1.0 / sqrt(n1x * n1x + n1y * n1y + n1z * n1z);
+    const double in2 = 1.0 / sqrt(n2x * n2x + n2y * n2y + n2z * n2z);
+    const double in3 = 1.0 / sqrt(n3x * n3x + n3y * n3y + n3z * n3z);
+    const double cp = (n1x * n2x + n1y * n2y + n1z * n2z) * in1 * in2;
+    const double sp = (n3x * n2x + n3y * n2y + n3z * n2z) * in3 * in2;
+    n2x *= in2;
+    n2y *= in2;
+    n2z *= in2;
+    const double phi = -atan2(sp, cp);
+    const bool use_cos = fabs(sp) > 0.1;
+    double c1x = 0, c1y = 0, c1z = 0, c2x = 0, c2y = 0, c2z = 0;
+    if (use_cos) {
+      n1x *= in1;
+      n1y *= in1;
+      n1z *= in1;
+      c1x = in1 * (n1x * cp - n2x);
+      c1y = in1 * (n1y * cp - n2y);
+      c1z = in1 * (n1z * cp - n2z);
+      c2x = in2 * (n2x * cp - n1x);
+      c2y = in2 * (n2y * cp - n1y);
+      c2z = in2 * (n2z * cp - n1z);
+    } else {
+      n3x *= in3;
+      n3y *= in3;
+      n3z *= in3;
+      c1x = in3 * (n3x * sp - n2x);  // dsin/dnrml3
+      c1y = in3 * (n3y * sp - n2y);
+      c1z = in3 * (n3z * sp - n2z);
+      c2x = in2 * (n2x * sp - n3x);  // dsin/dnrml2
+      c2y = in2 * (n2y * sp - n3y);
+      c2z = in2 * (n2z * sp - n3z);
+    }
+    double fix = 0, fiy = 0, fiz = 0, fabx = 0, faby = 0, fabz = 0, fjx = 0, fjy = 0, fjz = 0;
+    const int nterms = tterms[q];
+    for (int h = 0; h < nterms && h < 4; h++) {
+      const double kk = tvn[4 * q + h], nn = tnn[4 * q + h], g = tgamma[4 * q + h];
+      double ep, dEdphi;
+      if (nn > 0) {
+        ep = kk * (1.0 + cos(nn * phi + g));
+        dEdphi = -nn * kk * sin(nn * phi + g);
+      } else {
+        double diff = phi - g;
+        const double PI = 3.14159265358979323846;
+        if (diff < -PI)
+          diff += 2.0 * PI;
+        else if (diff > PI)
+          diff -= 2.0 * PI;
+        ep = kk * diff * diff;
+        dEdphi = 2.0 * kk * diff;
+      }
+      et += ep;
+      if (use_cos) {
+        dEdphi /= sp;
+        fix += dEdphi * (v2y * c1z - v2z * c1y);
+        fiy += dEdphi * (v2z * c1x - v2x * c1z);
+        fiz += dEdphi * (v2x * c1y - v2y * c1x);
+        fjx += dEdphi * (v2z * c2y - v2y * c2z);
+        fjy += dEdphi * (v2x * c2z - v2z * c2x);
+        fjz += dEdphi * (v2y * c2x - v2x * c2y);
+        fabx += dEdphi * (v1z * c1y - v1y * c1z + v3y * c2z - v3z * c2y);
+        faby += dEdphi * (v1x * c1z - v1z * c1x + v3z * c2x - v3x * c2z);
+        fabz += dEdphi * (v1y * c1x - v1x * c1y + v3x * c2y - v3y * c2x);
+      } else {
+        dEdphi /= -cp;
+        fix += dEdphi * ((v2y * v2y + v2z * v2z) * c1x - v2x * v2y * c1y - v2x * v2z * c1z);
+        fiy += dEdphi * ((v2z * v2z + v2x * v2x) * c1y - v2y * v2z * c1z - v2y * v2x * c1x);
+        fiz += dEdphi * ((v2x * v2x + v2y * v2y) * c1z - v2z * v2x * c1x - v2z * v2y * c1y);
+        fjx += dEdphi * (c2y * v2z - c2z * v2y);
+        fjy += dEdphi * (c2z * v2x - c2x * v2z);
+        fjz += dEdphi * (c2x * v2y - c2y * v2x);
+        fabx += dEdphi * (-(v2y * v1y + v2z * v1z) * c1x + (2.0 * v2x * v1y - v1x * v2y) * c1y +
+                          (2.0 * v2x * v1z - v1x * v2z) * c1z + c2z * v3y - c2y * v3z);
+        faby += dEdphi * (-(v2z * v1z + v2x * v1x) * c1y + (2.0 * v2y * v1z - v1y * v2z) * c1z +
+                          (2.0 * v2y * v1x - v1y * v2x) * c1x + c2x * v3z - c2z * v3x);
+        fabz += dEdphi * (-(v2x * v1x + v2y * v1y) * c1z + (2.0 * v2z * v1x - v1z * v2x) * c1x +
+                          (2.0 * v2z * v1y - v1z * v2y) * c1y + c2y * v3x - c2x * v3y);
+      }
+    }
+    add3(fx, fy, fz, i0, fix, fiy, fiz);
+    add3(fx, fy, fz, i1, fabx - fix, faby - fiy, fabz - fiz);
+    add3(fx, fy, fz, i2, fjx - fabx, fjy - faby, fjz - fabz);
+    add3(fx, fy, fz, i3, -fjx, -fjy, -fjz);
+  }
+  block_partial(eb, ea, et, vir, partial);
+}
+
+}  // namespace
+
+// Reduction kernel shared with the other components (k_nonbonded.hip).
+void reduce4(hipStream_t s, const double *part, int count, double *out);
+
+void Engine::launch_bonded(double *energy_out) {
+  const int total = nbonds + nangles + ntors;
+  const int T = 256;
+  const int grid = std::max(1, (total + T - 1) / T);
+  DBuf<double> &part = e_partial_bonded;
+  part.resize((size_t)grid * 4);
+  hipLaunchKernelGGL(k_bonded, dim3(grid), dim3(T), 0, stream, nbonds, b_i.p, b_j.p, b_l.p, b_k.p, nangles, a_i.p,
+                     a_a.p, a_j.p, a_t0.p, a_k.p, ntors, t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p,
+                     t_gamma.p, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, part.p);
+  PDGPU_HIP(hipGetLastError());
+  reduce4(stream, part.p, grid, energy_out);
+}
+
+}  // namespace pdgpu

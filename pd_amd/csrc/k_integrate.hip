@@ -1,0 +1,419 @@
+// k_integrate.hip -- force/energy collection and the device-resident MD loop.
+//
+// The integrators restate MolecularDynamics in FP64 and PD units (m/s, J/m, kg, s):
+//   velocity Verlet as PD implements it (md.cpp:805-855): v += F dt/2m, kinetic
+//   energy, then v += dv; r += dt (v + dv) / A; v += dv with the same old force;
+//   Langevin (md.cpp:923-1059) with PD's c0/c1/c2 coefficients and correlated
+//   position/velocity noise, drawing its unit normals from Philox4x32-10.
+// Expressions keep PD's evaluation order with FMA contraction off, so given the
+// same forces a step reproduces the oracle's FP64 arithmetic.
+#include "engine.hpp"
+
+#include <cmath>
+
+namespace pdgpu {
+
+namespace {
+
+__global__ void k_zero3(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c,
+                        int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    a[i] = 0;
+    b[i] = 0;
+    c[i] = 0;
+  }
+}
+
+__global__ void k_reduce4_kernel(const double *__restrict__ part, int count, double *__restrict__ out) {
+  __shared__ double s[4][256];
+  double a[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < count; i += 256)
+    for (int d = 0; d < 4; d++) a[d] += part[4 * i + d];
+  for (int d = 0; d < 4; d++) s[d][threadIdx.x] = a[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 4; d++) s[d][threadIdx.x] += s[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Total force of atom i in J/m: the original-order accumulator plus the
+// nonbonded slot-order accumulator of the atom's slot.
+__device__ __forceinline__ void total_force(int i, const long long *__restrict__ fbx,
+                                            const long long *__restrict__ fby, const long long *__restrict__ fbz,
+                                            const long long *__restrict__ nbx, const long long *__restrict__ nby,
+                                            const long long *__restrict__ nbz, const int *__restrict__ orig_slot,
+                                            double conv, double &f0, double &f1, double &f2) {
+  long long a = fbx[i], b = fby[i], c = fbz[i];
+  if (nbx) {
+    int s = orig_slot[i];
+    a += nbx[s];
+    b += nby[s];
+    c += nbz[s];
+  }
+  f0 = (double)a * conv;
+  f1 = (double)b * conv;
+  f2 = (double)c * conv;
+}
+
+__device__ void block_sum4(double v0, double *out) {
+  __shared__ double s[256];
+  s[threadIdx.x] = v0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[4 * blockIdx.x + 0] = s[0];
+    out[4 * blockIdx.x + 1] = 0.0;
+    out[4 * blockIdx.x + 2] = 0.0;
+    out[4 * blockIdx.x + 3] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_velocity_verlet(
+    int n, double t, double conv, const double *__restrict__ mass, const long long *__restrict__ fbx,
+    const long long *__restrict__ fby, const long long *__restrict__ fbz, const long long *__restrict__ nbx,
+    const long long *__restrict__ nby, const long long *__restrict__ nbz, const int *__restrict__ orig_slot,
+    double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, double *__restrict__ vx,
+    double *__restrict__ vy, double *__restrict__ vz, double *__restrict__ ke_partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < n) {
+    double f0, f1, f2;
+    total_force(i, fbx, fby, fbz, nbx, nby, nbz, orig_slot, conv, f0, f1, f2);
+    const double m = mass[i];
+    const double tinvmass = 0.5 * t / m;
+    double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+    v0 += f0 * tinvmass;  // first loop, md.cpp:820-829
+    v1 += f1 * tinvmass;
+    v2 += f2 * tinvmass;
+    const double mag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);  // calcKineticEnergy, md.cpp:530
+    ke = 0.5 * m * (mag * mag);
+    const double ia = 1 / Consts::Angstrom;
+    const double d0 = f0 * tinvmass, d1 = f1 * tinvmass, d2 = f2 * tinvmass;  // second loop, md.cpp:838-853
+    v0 += d0;
+    v1 += d1;
+    v2 += d2;
+    double r0 = t * (v0 + d0), r1 = t * (v1 + d1), r2 = t * (v2 + d2);
+    r0 *= ia;
+    r1 *= ia;
+    r2 *= ia;
+    x[i] += r0;
+    y[i] += r1;
+    z[i] += r2;
+    vx[i] = v0 + d0;
+    vy[i] = v1 + d1;
+    vz[i] = v2 + d2;
+  }
+  block_sum4(ke, ke_partial);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// Six unit normals per atom and step, in the order of PD's three nrand calls
+// (md.cpp:1011-1013): n1x n2y | n2x n1z | n1y n2z, each pair from Box-Muller
+// in the form of maths.h:155-169.
+__device__ void langevin_noise(uint32_t k0, uint32_t k1, uint32_t stream, uint32_t step, uint32_t atom,
+                               double out[6]) {
+#pragma clang fp contract(off)
+  const double TwoPI = 2.0 * Consts::PI;
+  for (int k = 0; k < 3; k++) {
+    uint32_t c[4] = {atom, step, stream, (uint32_t)k};
+    philox(c, k0, k1);
+    const double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]);
+    const double l = sqrt(-2.0 * log(u1));
+    out[2 * k] = l * cos((-TwoPI) * u2);
+    out[2 * k + 1] = l * sin((-TwoPI) * u2);
+  }
+}
+
+struct LangevinArgs {
+  double t, gamma, gammat, temp;
+  double c0, c1, c2, varr, varv, crv;
+  int first_kick;   // Step > 0
+  int move;         // Step < Steps-1
+  uint32_t k0, k1, stream, step;
+  double conv;
+};
+
+__global__ __launch_bounds__(256) void k_langevin(
+    int n, LangevinArgs L, const double *__restrict__ mass, const long long *__restrict__ fbx,
+    const long long *__restrict__ fby, const long long *__restrict__ fbz, const long long *__restrict__ nbx,
+    const long long *__restrict__ nby, const long long *__restrict__ nbz, const int *__restrict__ orig_slot,
+    double *__restrict__ x, double *__restrict__ y, double *__restrict__ z, double *__restrict__ vx,
+    double *__restrict__ vy, double *__restrict__ vz, double *__restrict__ ke_partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < n) {
+    double f[3];
+    total_force(i, fbx, fby, fbz, nbx, nby, nbz, orig_slot, L.conv, f[0], f[1], f[2]);
+    const double m = mass[i];
+    const double t = L.t;
+    const double tinvmass = t / m;
+    double v[3] = {vx[i], vy[i], vz[i]};
+    if (L.first_kick) {
+      for (int d = 0; d < 3; d++) v[d] += (1.0 - L.c0 / L.c1) * f[d] * tinvmass / L.gammat;
+    }
+    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    ke = 0.5 * m * (mag * mag);
+    if (L.move) {
+      const double kTdivm = Consts::kB * L.temp / m;
+      const double sigmav = sqrt(kTdivm * L.varv);
+      const double sigmar = sqrt(kTdivm * L.varr) / L.gamma;
+      double z6[6];
+      langevin_noise(L.k0, L.k1, L.stream, L.step, (uint32_t)i, z6);
+      const double n1[3] = {z6[0], z6[4], z6[3]};  // n1x n1y n1z
+      const double n2[3] = {z6[2], z6[1], z6[5]};  // n2x n2y n2z
+      const double s = sqrt(1.0 - L.crv * L.crv);
+      const double ia = 1 / Consts::Angstrom;
+      double dr[3], dv[3];
+      for (int d = 0; d < 3; d++) {
+        const double delv = sigmav * (L.crv * n1[d] + s * n2[d]);
+        const double delr = sigmar * n1[d];
+        dr[d] = t * (L.c1 * v[d] + L.c2 * f[d] * tinvmass) + delr;
+        dr[d] *= ia;
+        dv[d] = (L.c0 * L.c2) * f[d] * tinvmass / L.c1 + delv;
+      }
+      x[i] += dr[0];
+      y[i] += dr[1];
+      z[i] += dr[2];
+      for (int d = 0; d < 3; d++) {
+        v[d] *= L.c0;
+        v[d] += dv[d];
+      }
+    }
+    vx[i] = v[0];
+    vy[i] = v[1];
+    vz[i] = v[2];
+  }
+  block_sum4(ke, ke_partial);
+}
+
+// trace[2*s] = epot (J), trace[2*s+1] = ekin (J); energies are in kcal/mol
+// in e_out (4 doubles per component slot).
+__global__ void k_record(const double *__restrict__ e_out, int use_nb, int use_bonded, int use_gb, double vdwcor,
+                         double kcal_to_J, double *__restrict__ trace, int s) {
+  if (threadIdx.x != 0) return;
+  double ep = 0.0;
+  if (use_nb) ep += e_out[0] + e_out[1] + vdwcor;
+  if (use_bonded) ep += e_out[4] + e_out[5] + e_out[6];
+  if (use_gb) ep += e_out[8] + e_out[9] + e_out[10] + e_out[11];
+  trace[2 * s] = ep * kcal_to_J;
+  trace[2 * s + 1] = e_out[12];
+}
+
+}  // namespace
+
+void reduce4(hipStream_t s, const double *part, int count, double *out) {
+  hipLaunchKernelGGL(k_reduce4_kernel, dim3(1), dim3(256), 0, s, part, count, out);
+}
+
+// e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
+// bond, angle, torsion, virial; [8..11] GB pol, vdw, elec, virial (see k_gb.hip);
+// [12] kinetic energy (J); [13..15] spare.
+void Engine::zero_forces() {
+  fbx.resize(n);
+  fby.resize(n);
+  fbz.resize(n);
+  hipLaunchKernelGGL(k_zero3, dim3((n + 255) / 256), dim3(256), 0, stream, fbx.p, fby.p, fbz.p, n);
+}
+
+void Engine::evaluate(uint32_t components) {
+  e_out.resize(16);
+  PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 16 * sizeof(double), stream));
+  zero_forces();
+  if (components & PDGPU_NONBONDED) {
+    if (!list_valid) build_list();
+    launch_nonbonded(e_out.p + 0);
+  }
+  if ((components & PDGPU_BONDED) && (nbonds + nangles + ntors) > 0) launch_bonded(e_out.p + 4);
+  if (components & PDGPU_GB) launch_gb(e_out.p + 8);
+  last_components = components;
+}
+
+void Engine::forces_to_host(double *f, size_t stride_bytes) {
+  std::vector<long long> a(n), b(n), c(n);
+  fbx.download(a.data(), n, stream);
+  fby.download(b.data(), n, stream);
+  fbz.download(c.data(), n, stream);
+  std::vector<long long> sa, sb, sc;
+  std::vector<int> os;
+  const bool nbon = (last_components & PDGPU_NONBONDED) != 0;
+  if (nbon) {
+    const size_t ns = (size_t)max_clusters * kClusterSize;
+    sa.resize(ns);
+    sb.resize(ns);
+    sc.resize(ns);
+    os.resize(n);
+    nbx.download(sa.data(), ns, stream);
+    nby.download(sb.data(), ns, stream);
+    nbz.download(sc.data(), ns, stream);
+    orig_slot.download(os.data(), n, stream);
+  }
+  sync();
+  const double conv = Consts::kcalA_to_Jm() / kFixScale;
+  for (int i = 0; i < n; i++) {
+    long long fx = a[i], fy = b[i], fz = c[i];
+    if (nbon) {
+      fx += sa[os[i]];
+      fy += sb[os[i]];
+      fz += sc[os[i]];
+    }
+    double *p = reinterpret_cast<double *>(reinterpret_cast<char *>(f) + (size_t)i * stride_bytes);
+    p[0] += (double)fx * conv;
+    p[1] += (double)fy * conv;
+    p[2] += (double)fz * conv;
+  }
+}
+
+void Engine::energies_to_host(uint32_t components, pdgpu_energies *out) {
+  double h[16];
+  e_out.download(h, 16, stream);
+  sync();
+  const double kj = Consts::kcal_to_J();
+  pdgpu_energies e{};
+  if (components & PDGPU_NONBONDED) {
+    const double cor = nbk.vdwcor_per_atom * n;
+    e.epot_vdw += (h[0] + cor) * kj;
+    e.epot_elec += h[1] * kj;
+    e.virial += h[2] * kj;
+    e.epot += (h[0] + h[1] + cor) * kj;
+  }
+  if (components & PDGPU_BONDED) {
+    e.epot_bond += h[4] * kj;
+    e.epot_angle += h[5] * kj;
+    e.epot_torsion += h[6] * kj;
+    e.virial += h[7] * kj;
+    e.epot += (h[4] + h[5] + h[6]) * kj;
+  }
+  if (components & PDGPU_GB) gb_energies_to_host(h + 8, e);
+  *out = e;
+}
+
+void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
+  const int T = 256;
+  const int grid = (n + T - 1) / T;
+  e_partial_ke.resize((size_t)grid * 4);
+  const bool nbon = (last_components & PDGPU_NONBONDED) != 0;
+  const long long *ax = nbon ? nbx.p : nullptr, *ay = nbon ? nby.p : nullptr, *az = nbon ? nbz.p : nullptr;
+  const double conv = Consts::kcalA_to_Jm() / kFixScale;
+  const double gammat = p.fric_coeff * p.timestep;
+  if (p.integrator == PDGPU_VELOCITY_VERLET || gammat < 0.00001) {
+    // Langevin with gamma*dt < 1e-5 falls back to velocity Verlet (md.cpp:966-971)
+    hipLaunchKernelGGL(k_velocity_verlet, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, mass.p, fbx.p,
+                       fby.p, fbz.p, ax, ay, az, orig_slot.p, x.p, y.p, z.p, vx.p, vy.p, vz.p, e_partial_ke.p);
+  } else {
+    LangevinArgs L{};
+    L.t = p.timestep;
+    L.gamma = p.fric_coeff;
+    L.gammat = gammat;
+    L.temp = p.temperature;
+    langevin_coefficients(gammat, &L.c0);
+    L.first_kick = step > 0 ? 1 : 0;
+    L.move = step < (p.total_steps - 1) ? 1 : 0;
+    L.k0 = (uint32_t)p.seed;
+    L.k1 = (uint32_t)(p.seed >> 32);
+    L.stream = 0;
+    L.step = (uint32_t)step;
+    L.conv = conv;
+    hipLaunchKernelGGL(k_langevin, dim3(grid), dim3(T), 0, stream, n, L, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az,
+                       orig_slot.p, x.p, y.p, z.p, vx.p, vy.p, vz.p, e_partial_ke.p);
+  }
+  PDGPU_HIP(hipGetLastError());
+  reduce4(stream, e_partial_ke.p, grid, ke_slot);
+}
+
+// c0 c1 c2 varr varv crv of applyForces_LangevinIntegration (md.cpp:947-986).
+void langevin_coefficients(double gammat, double *c) {
+  const double gt = gammat, gt2 = gt * gt, gt3 = gt * gt2, gt4 = gt2 * gt2, gt5 = gt2 * gt3, gt6 = gt3 * gt3,
+               gt7 = gt3 * gt4, gt8 = gt4 * gt4, gt9 = gt4 * gt5;
+  double c0, c1, c2, varr, varv, crv;
+  if (gammat > 0.05) {
+    c0 = std::exp(-gammat);
+    c1 = (1 - c0) / gammat;
+    c2 = (1 - c1) / gammat;
+    const double eg = std::exp(-gammat);
+    varv = (1.0 - eg * eg);
+    varr = (2.0 * gammat - 3.0 + (4.0 - eg) * eg);
+    crv = (1.0 - eg) * (1.0 - eg) / (std::sqrt(varv * varr));
+  } else {
+    c0 = 1 - gt + 0.5 * gt2 - gt3 / 6.0 + gt4 / 24.0 - gt5 / 120.0 + gt6 / 720.0 - gt7 / 5040.0 + gt8 / 40320.0 -
+         gt9 / 362880.0;
+    c1 = 1 - (0.5 * gt - gt2 / 6.0 + gt3 / 24.0 - gt4 / 120.0 + gt5 / 720.0 - gt6 / 5040.0 + gt7 / 40320.0 -
+              gt8 / 362880.0);
+    c2 = 0.5 - gt / 6.0 + gt2 / 24.0 - gt3 / 120.0 + gt4 / 720.0 - gt5 / 5040.0 + gt6 / 40320.0 - gt7 / 362880.0;
+    varr = 2.0 * gt3 / 3.0 - gt4 / 2.0 + 7.0 * gt5 / 30.0 - gt6 / 12.0 + 31.0 * gt7 / 1260.0 - gt8 / 160.0 +
+           127.0 * gt9 / 90720.0;
+    varv = 2.0 * gt - 2.0 * gt2 + 4.0 * gt3 / 3.0 - 2.0 * gt4 / 3.0 + 4.0 * gt5 / 15.0 - 4.0 * gt6 / 45.0 +
+           8.0 * gt7 / 315.0 - 2.0 * gt8 / 315.0 + 4.0 * gt9 / 2835.0;
+    crv = std::sqrt(3.0) * (0.5 - 3.0 * gt / 16.0 - 17.0 * gt2 / 1280.0 + 17.0 * gt3 / 6144.0 +
+                            40967.0 * gt4 / 34406400.0 - 57203.0 * gt5 / 275251200.0 -
+                            1429487.0 * gt6 / 13212057600.0);
+  }
+  c[0] = c0;
+  c[1] = c1;
+  c[2] = c2;
+  c[3] = varr;
+  c[4] = varv;
+  c[5] = crv;
+}
+
+// MolecularDynamics::run_core (md.cpp:219-284): refresh the list every
+// UpdateNList steps (protocolbase.cpp:135-144), evaluate forces, integrate.
+void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, double *ekin_trace) {
+  require(have_positions, "positions not set");
+  require(vx.n >= (size_t)n && mass.n >= (size_t)n, "velocities or masses not set");
+  require(p.timestep > 0, "timestep must be positive");
+  trace.resize((size_t)std::max(n_steps, 1) * 2);
+  const uint32_t comp = p.components;
+  const double cor = (comp & PDGPU_NONBONDED) ? nbk.vdwcor_per_atom * n : 0.0;
+  for (int s = 0; s < n_steps; s++) {
+    const int step = p.step_offset + s;
+    if ((comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0) build_list();
+    evaluate(comp);
+    integrate(p, step, e_out.p + 12);
+    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, stream, e_out.p, (comp & PDGPU_NONBONDED) ? 1 : 0,
+                       (comp & PDGPU_BONDED) ? 1 : 0, (comp & PDGPU_GB) ? 1 : 0, cor, Consts::kcal_to_J(),
+                       trace.p, s);
+  }
+  std::vector<double> h((size_t)n_steps * 2);
+  trace.download(h.data(), h.size(), stream);
+  sync();
+  bool nan = false;
+  for (int s = 0; s < n_steps; s++) {
+    if (epot_trace) epot_trace[s] = h[2 * s];
+    if (ekin_trace) ekin_trace[s] = h[2 * s + 1];
+    if (std::isnan(h[2 * s])) nan = true;
+  }
+  if (nan) throw EngineError(PDGPU_ERR_UNSTABLE, "Simulation unstable or division by zero");
+}
+
+}  // namespace pdgpu

@@ -1,0 +1,589 @@
+// k_list.hip -- WorkSpace::cleanSpace + NeighbourList::calcNewList on the device.
+//
+// PD builds the list with an O(N^2) double loop (neighbourlist.cpp:336-439 for
+// PeriodicBox, 247-333 for InfiniteSpace).  Here atoms are binned into grid
+// columns, sorted by z inside each column and cut into clusters of 8 slots.
+// For every i-cluster one wavefront finds the j-clusters (index >= i-cluster)
+// whose bounding boxes come within Cutoff+Padding, then tests every atom pair
+// with PD's own FP64 arithmetic: d = p_j - p_i, wrapped by the `while` minimum-
+// image loops that also produce the image number, r^2 = (x*x + y*y) + z*z with
+// no FMA, kept when r^2 <= (Cutoff+Padding)^2.  The result is the exact PD pair
+// set, stored per i-cluster as 8-byte entries:
+//   x: j slot (24 bits) | image (5 bits) << 24    image as seen from the i atom
+//   y: listed mask over the 8 i atoms (8 bits) | 2-bit bond-order code per i atom
+//      << 8 (0 = order 7, 1 = 1-2, 2 = 1-3, 3 = 1-4)
+// Bond orders follow the list builder: the band matrix for |i-j| < MaxIndexDelta,
+// 7 otherwise, then the off-diagonal corrections (neighbourlist.cpp:179-220).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace pdgpu {
+
+namespace {
+
+constexpr int kMaxCand = 1536;  // candidate j-clusters per i-cluster (LDS)
+
+__global__ void k_clean_space(int n_mol, const int *__restrict__ first, const int *__restrict__ last,
+                              double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                              double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  // WorkSpace::cleanSpace (workspace.cpp:371-380); centre of geometry as
+  // MoleculeBase::getCentreOfGeometry(ifirst, ilast) (molecule.cpp:850-858).
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol) return;
+  const double hx = bx / 2.0, hy = by / 2.0, hz = bz / 2.0;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  const int f = first[m], l = last[m];
+  for (int i = f; i <= l; i++) {
+    cx += x[i];
+    cy += y[i];
+    cz += z[i];
+  }
+  const double cnt = (double)(l - f + 1);
+  cx /= cnt;
+  cy /= cnt;
+  cz /= cnt;
+  double ix = cx, iy = cy, iz = cz;  // PeriodicBox::getClosestImage (space.h:215-227)
+  for (int it = 0; it < 64 && ix > hx; it++) ix -= bx;
+  for (int it = 0; it < 64 && ix < -hx; it++) ix += bx;
+  for (int it = 0; it < 64 && iy > hy; it++) iy -= by;
+  for (int it = 0; it < 64 && iy < -hy; it++) iy += by;
+  for (int it = 0; it < 64 && iz > hz; it++) iz -= bz;
+  for (int it = 0; it < 64 && iz < -hz; it++) iz += bz;
+  const double sx = ix - cx, sy = iy - cy, sz = iz - cz;
+  for (int i = f; i <= l; i++) {
+    x[i] += sx;
+    y[i] += sy;
+    z[i] += sz;
+  }
+}
+
+__global__ void k_bbox(int n, const double *__restrict__ x, const double *__restrict__ y,
+                       const double *__restrict__ z, double *__restrict__ out) {
+  __shared__ double s[6][1024];
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lo[0] = fmin(lo[0], x[i]); hi[0] = fmax(hi[0], x[i]);
+    lo[1] = fmin(lo[1], y[i]); hi[1] = fmax(hi[1], y[i]);
+    lo[2] = fmin(lo[2], z[i]); hi[2] = fmax(hi[2], z[i]);
+  }
+  for (int d = 0; d < 3; d++) {
+    s[d][threadIdx.x] = lo[d];
+    s[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) {
+        s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + w]);
+        s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+struct GridArgs {
+  int periodic;
+  int ncx, ncy;
+  double lo_x, lo_y, w_x, w_y;   // column origin and width
+  double L[3];                   // box (periodic)
+};
+
+__device__ __forceinline__ double wrap_coord(double v, double L) {
+  // representative in [-L/2, L/2) used for binning only
+  return v - L * floor(v / L + 0.5);
+}
+
+__global__ void k_bin(int n, GridArgs g, const double *__restrict__ x, const double *__restrict__ y,
+                      const double *__restrict__ z, double *__restrict__ wx, double *__restrict__ wy,
+                      double *__restrict__ wz, int *__restrict__ atom_col, int *__restrict__ col_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = x[i], b = y[i], c = z[i];
+  if (g.periodic) {
+    a = wrap_coord(a, g.L[0]);
+    b = wrap_coord(b, g.L[1]);
+    c = wrap_coord(c, g.L[2]);
+  }
+  wx[i] = a;
+  wy[i] = b;
+  wz[i] = c;
+  int cx = (int)floor((a - g.lo_x) / g.w_x);
+  int cy = (int)floor((b - g.lo_y) / g.w_y);
+  cx = min(max(cx, 0), g.ncx - 1);
+  cy = min(max(cy, 0), g.ncy - 1);
+  int col = cy * g.ncx + cx;
+  atom_col[i] = col;
+  atomicAdd(&col_count[col], 1);
+}
+
+// Single-workgroup exclusive scans over the columns.
+__global__ void k_scan_columns(int ncol, const int *__restrict__ col_count, int *__restrict__ col_atom_start,
+                               int *__restrict__ col_cl_start, int *__restrict__ col_ncl,
+                               int *__restrict__ col_fill, int *__restrict__ n_clusters) {
+  __shared__ int sa[1024], sc[1024];
+  __shared__ int carry_a, carry_c;
+  if (threadIdx.x == 0) {
+    carry_a = 0;
+    carry_c = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < ncol; base += 1024) {
+    int c = base + threadIdx.x;
+    int cnt = c < ncol ? col_count[c] : 0;
+    int ncl = (cnt + kClusterSize - 1) / kClusterSize;
+    sa[threadIdx.x] = cnt;
+    sc[threadIdx.x] = ncl;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+      int va = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0;
+      int vc = threadIdx.x >= (unsigned)off ? sc[threadIdx.x - off] : 0;
+      __syncthreads();
+      sa[threadIdx.x] += va;
+      sc[threadIdx.x] += vc;
+      __syncthreads();
+    }
+    if (c < ncol) {
+      col_atom_start[c] = carry_a + sa[threadIdx.x] - cnt;
+      col_cl_start[c] = carry_c + sc[threadIdx.x] - ncl;
+      col_ncl[c] = ncl;
+      col_fill[c] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) {
+      carry_a += sa[1023];
+      carry_c += sc[1023];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_clusters = carry_c;
+}
+
+__global__ void k_fill_columns(int n, const int *__restrict__ atom_col, const int *__restrict__ col_atom_start,
+                               int *__restrict__ col_fill, int *__restrict__ col_atoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = atom_col[i];
+  int k = atomicAdd(&col_fill[c], 1);
+  col_atoms[col_atom_start[c] + k] = i;
+}
+
+// Rank of each atom inside its column by (wrapped z, atom index): a total order,
+// so the slot assignment does not depend on the atomic fill order.
+__global__ void k_sort_columns(int n, const int *__restrict__ atom_col, const int *__restrict__ col_count,
+                               const int *__restrict__ col_atom_start, const int *__restrict__ col_cl_start,
+                               const int *__restrict__ col_atoms, const double *__restrict__ wz,
+                               int *__restrict__ slot_orig, int *__restrict__ orig_slot,
+                               int *__restrict__ cl_col) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int c = atom_col[a];
+  const int s = col_atom_start[c], cnt = col_count[c];
+  const double za = wz[a];
+  int rank = 0;
+  for (int k = 0; k < cnt; k++) {
+    int b = col_atoms[s + k];
+    double zb = wz[b];
+    rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
+  }
+  int slot = col_cl_start[c] * kClusterSize + rank;
+  slot_orig[slot] = a;
+  orig_slot[a] = slot;
+  cl_col[slot / kClusterSize] = c;
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int v, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+__global__ void k_cluster_bbox(const int *__restrict__ n_clusters, int max_clusters,
+                               const int *__restrict__ slot_orig, const double *__restrict__ wx,
+                               const double *__restrict__ wy, const double *__restrict__ wz,
+                               double *__restrict__ lo, double *__restrict__ hi) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= max_clusters || c >= *n_clusters) return;
+  double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+  for (int k = 0; k < kClusterSize; k++) {
+    int a = slot_orig[c * kClusterSize + k];
+    if (a < 0) continue;
+    l[0] = fmin(l[0], wx[a]); h[0] = fmax(h[0], wx[a]);
+    l[1] = fmin(l[1], wy[a]); h[1] = fmax(h[1], wy[a]);
+    l[2] = fmin(l[2], wz[a]); h[2] = fmax(h[2], wz[a]);
+  }
+  for (int d = 0; d < 3; d++) {
+    lo[3 * c + d] = l[d];
+    hi[3 * c + d] = h[d];
+  }
+}
+
+struct SearchArgs {
+  int periodic;
+  int ncx, ncy;
+  int Dx, Dy;                   // column search half-widths
+  double L[3];
+  double half[3];
+  double rlist2;                // (Cutoff+Padding)^2, PD's sqrIncludeLimit
+  double Rbox2;                 // squared bbox pruning radius (rlist plus a margin)
+  int mid;                      // BondOrder::m_MaxIndexDelta
+  int stride;                   // entries reserved per i-cluster
+};
+
+__device__ __forceinline__ int bond_code(int oa, int ob, int mid, const int8_t *__restrict__ band,
+                                         const int *__restrict__ sp_off, const int *__restrict__ sp_part,
+                                         const int *__restrict__ sp_ord) {
+  int hi = max(oa, ob), lo = min(oa, ob);
+  int bo = 7;
+  if (hi - lo < mid) bo = band[(long)lo * mid + (hi - lo - 1)];
+  // off-diagonal corrections (neighbourlist.cpp:179-220): the last partner entry wins
+  for (int k = sp_off[lo]; k < sp_off[lo + 1]; k++)
+    if (sp_part[k] == hi) bo = sp_ord[k];
+  bo &= 7;
+  return bo == 1 ? 1 : bo == 2 ? 2 : bo == 3 ? 3 : 0;
+}
+
+__device__ __forceinline__ unsigned spread8(unsigned m) {
+  m = (m | (m << 4)) & 0x0F0Fu;
+  m = (m | (m << 2)) & 0x3333u;
+  m = (m | (m << 1)) & 0x5555u;
+  return m;
+}
+
+__global__ __launch_bounds__(64) void k_pair_search(
+    SearchArgs A, const int *__restrict__ n_clusters, const int *__restrict__ cl_col,
+    const int *__restrict__ col_cl_start, const int *__restrict__ col_ncl,
+    const double *__restrict__ cl_lo, const double *__restrict__ cl_hi, const int *__restrict__ slot_orig,
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ wx, const double *__restrict__ wy, const double *__restrict__ wz,
+    const int8_t *__restrict__ band, const int *__restrict__ sp_off, const int *__restrict__ sp_part,
+    const int *__restrict__ sp_ord, uint2 *__restrict__ entries, int *__restrict__ cl_entry_count,
+    unsigned long long *__restrict__ counter) {
+#pragma clang fp contract(off)
+  __shared__ int cand_cl[kMaxCand];
+  __shared__ int cand_sh[kMaxCand];
+  const int ci = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (ci >= *n_clusters) return;
+
+  const double ilo0 = cl_lo[3 * ci], ilo1 = cl_lo[3 * ci + 1], ilo2 = cl_lo[3 * ci + 2];
+  const double ihi0 = cl_hi[3 * ci], ihi1 = cl_hi[3 * ci + 1], ihi2 = cl_hi[3 * ci + 2];
+  const int col = cl_col[ci];
+  const int cx = col % A.ncx, cy = col / A.ncx;
+
+  // ---- phase 1: candidate j-clusters, deterministic order -------------------
+  int ncand = 0;
+  bool cand_overflow = false;
+  const int kz_lo = A.periodic ? -1 : 0, kz_hi = A.periodic ? 1 : 0;
+  for (int dy = -A.Dy; dy <= A.Dy; dy++) {
+    int uy = cy + dy, ky = 0;
+    if (A.periodic) {
+      for (int it = 0; it < 4 && uy < 0; it++) { uy += A.ncy; ky -= 1; }
+      for (int it = 0; it < 4 && uy >= A.ncy; it++) { uy -= A.ncy; ky += 1; }
+      if (ky < -1 || ky > 1 || uy < 0 || uy >= A.ncy) continue;
+    } else if (uy < 0 || uy >= A.ncy) {
+      continue;
+    }
+    for (int dx = -A.Dx; dx <= A.Dx; dx++) {
+      int ux = cx + dx, kx = 0;
+      if (A.periodic) {
+        for (int it = 0; it < 4 && ux < 0; it++) { ux += A.ncx; kx -= 1; }
+        for (int it = 0; it < 4 && ux >= A.ncx; it++) { ux -= A.ncx; kx += 1; }
+        if (kx < -1 || kx > 1 || ux < 0 || ux >= A.ncx) continue;
+      } else if (ux < 0 || ux >= A.ncx) {
+        continue;
+      }
+      const int jc = uy * A.ncx + ux;
+      const int cs = col_cl_start[jc], nc = col_ncl[jc];
+      const double shx = kx * A.L[0], shy = ky * A.L[1];
+      for (int kz = kz_lo; kz <= kz_hi; kz++) {
+        const double shz = kz * A.L[2];
+        for (int base = 0; base < nc; base += 64) {
+          int t = base + lane;
+          int cj = cs + t;
+          bool ok = (t < nc) && (cj >= ci);
+          if (ok) {
+            double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ihi0, ilo0 - (cl_hi[3 * cj] + shx)));
+            double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ihi1, ilo1 - (cl_hi[3 * cj + 1] + shy)));
+            double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ihi2, ilo2 - (cl_hi[3 * cj + 2] + shz)));
+            ok = (g0 * g0 + g1 * g1 + g2 * g2) <= A.Rbox2;
+          }
+          unsigned long long b = __ballot(ok);
+          int pre = __popcll(b & ((1ull << lane) - 1ull));
+          if (ok) {
+            int pos = ncand + pre;
+            if (pos < kMaxCand) {
+              cand_cl[pos] = cj;
+              cand_sh[pos] = (kx + 1) * 9 + (ky + 1) * 3 + (kz + 1);
+            }
+          }
+          ncand += __popcll(b);
+        }
+      }
+    }
+  }
+  if (ncand > kMaxCand) {
+    cand_overflow = true;
+    ncand = kMaxCand;
+  }
+  __syncthreads();
+
+  // ---- phase 2: exact PD pair tests, 8 i atoms x 8 j atoms per candidate -----
+  const int ii = lane & 7, jj = lane >> 3;
+  const int sa = ci * kClusterSize + ii;
+  const int oa = slot_orig[sa];
+  double pax = 0, pay = 0, paz = 0, wax = 0, way = 0, waz = 0;
+  if (oa >= 0) {
+    pax = x[oa]; pay = y[oa]; paz = z[oa];
+    wax = wx[oa]; way = wy[oa]; waz = wz[oa];
+  }
+  uint2 *E = entries + (size_t)ci * A.stride;
+  int cnt = 0;
+  for (int k = 0; k < ncand; k++) {
+    const int cj = cand_cl[k];
+    const int csh = cand_sh[k];
+    const int sb = cj * kClusterSize + jj;
+    const int ob = slot_orig[sb];
+    bool in = (oa >= 0) && (ob >= 0) && (cj != ci || sa < sb);
+    int img = 13;
+    int code = 0;
+    if (in) {
+      // dedup across candidate shifts: the wrapped-frame minimum image of this
+      // pair must be the candidate's shift
+      if (A.periodic) {
+        double ex = wx[ob] - wax, ey = wy[ob] - way, ez = wz[ob] - waz;
+        int kxw = 0, kyw = 0, kzw = 0;
+        for (int it = 0; it < 8 && ex > A.half[0]; it++) { ex -= A.L[0]; kxw -= 1; }
+        for (int it = 0; it < 8 && ex < -A.half[0]; it++) { ex += A.L[0]; kxw += 1; }
+        for (int it = 0; it < 8 && ey > A.half[1]; it++) { ey -= A.L[1]; kyw -= 1; }
+        for (int it = 0; it < 8 && ey < -A.half[1]; it++) { ey += A.L[1]; kyw += 1; }
+        for (int it = 0; it < 8 && ez > A.half[2]; it++) { ez -= A.L[2]; kzw -= 1; }
+        for (int it = 0; it < 8 && ez < -A.half[2]; it++) { ez += A.L[2]; kzw += 1; }
+        in = ((kxw + 1) * 9 + (kyw + 1) * 3 + (kzw + 1)) == csh;
+      }
+    }
+    if (in) {
+      // PD's arithmetic: dc = p_j - p_i, while-loop wrapping (neighbourlist.cpp:383-392)
+      double dx = x[ob] - pax, dy = y[ob] - pay, dz = z[ob] - paz;
+      if (A.periodic) {
+        for (int it = 0; it < 8 && dx > A.half[0]; it++) { dx -= A.L[0]; img -= 9; }
+        for (int it = 0; it < 8 && dx < -A.half[0]; it++) { dx += A.L[0]; img += 9; }
+        for (int it = 0; it < 8 && dy > A.half[1]; it++) { dy -= A.L[1]; img -= 3; }
+        for (int it = 0; it < 8 && dy < -A.half[1]; it++) { dy += A.L[1]; img += 3; }
+        for (int it = 0; it < 8 && dz > A.half[2]; it++) { dz -= A.L[2]; img -= 1; }
+        for (int it = 0; it < 8 && dz < -A.half[2]; it++) { dz += A.L[2]; img += 1; }
+      }
+      double r2 = dx * dx + dy * dy + dz * dz;
+      in = r2 <= A.rlist2;
+      if (in) code = bond_code(oa, ob, A.mid, band, sp_off, sp_part, sp_ord);
+    }
+    // emit one entry per (j atom, image); usually one round
+    unsigned long long remaining = __ballot(in);
+    while (remaining) {  // each round clears at least one bit
+      unsigned grp = (unsigned)((remaining >> (jj * 8)) & 0xFFull);
+      int leader = grp ? (jj * 8 + __ffs(grp) - 1) : lane;
+      int limg = __shfl(img, leader, 64);
+      bool match = in && ((remaining >> lane) & 1ull) && img == limg;
+      unsigned long long mb = __ballot(match);
+      unsigned long long lob = __ballot(match && (code & 1));
+      unsigned long long hib = __ballot(match && (code & 2));
+      unsigned gmask = 0;
+#pragma unroll
+      for (int g = 0; g < 8; g++) gmask |= (((mb >> (g * 8)) & 0xFFull) ? 1u : 0u) << g;
+      if (ii == 0 && ((gmask >> jj) & 1u)) {
+        int pos = cnt + __popc(gmask & ((1u << jj) - 1u));
+        unsigned lm = (unsigned)((mb >> (jj * 8)) & 0xFFull);
+        unsigned codes = spread8((unsigned)((lob >> (jj * 8)) & 0xFFull)) |
+                         (spread8((unsigned)((hib >> (jj * 8)) & 0xFFull)) << 1);
+        if (pos < A.stride) E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | (codes << 8));
+      }
+      cnt += __popc(gmask);
+      remaining &= ~mb;
+    }
+  }
+  if (lane == 0) {
+    cl_entry_count[ci] = min(cnt, A.stride);
+    if (cnt > A.stride || cand_overflow) atomicMax(&counter[1], (unsigned long long)max(cnt, 1));
+    atomicAdd(&counter[0], (unsigned long long)cnt);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+void Engine::build_list() {
+  require(have_positions, "positions not set");
+  require(list_cutoff > 0.0, "no cutoff requested from the neighbour list");
+  const int T = 256;
+  const int nb_atoms = (n + T - 1) / T;
+  // 1. cleanSpace (PeriodicBox only; InfiniteSpace::moveIntoBox is a no-op)
+  if (periodic && !h_mol_first.empty()) {
+    int nm = (int)h_mol_first.size();
+    hipLaunchKernelGGL(k_clean_space, dim3((nm + T - 1) / T), dim3(T), 0, stream, nm, mol_first.p,
+                       mol_last.p, x.p, y.p, z.p, box[0], box[1], box[2]);
+  }
+  // 2. grid of columns
+  const double rlist = list_cutoff + padding;
+  GridArgs g{};
+  g.periodic = periodic ? 1 : 0;
+  double ext[3];
+  if (periodic) {
+    for (int d = 0; d < 3; d++) {
+      g.L[d] = box[d];
+      ext[d] = box[d];
+    }
+    g.lo_x = -box[0] / 2.0;
+    g.lo_y = -box[1] / 2.0;
+  } else {
+    DBuf<double> bb;
+    bb.resize(6);
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, n, x.p, y.p, z.p, bb.p);
+    double hb[6];
+    bb.download(hb, 6, stream);
+    sync();
+    for (int d = 0; d < 3; d++) ext[d] = std::max(hb[3 + d] - hb[d], 1e-3);
+    g.lo_x = hb[0];
+    g.lo_y = hb[1];
+    for (int d = 0; d < 3; d++) g.L[d] = 0.0;
+  }
+  const double vol = ext[0] * ext[1] * ext[2];
+  const double rho = std::max((double)n / vol, 1e-4);
+  const double w_target = std::cbrt((double)kClusterSize / rho);
+  g.ncx = std::max(1, std::min(4096, (int)(ext[0] / w_target)));
+  g.ncy = std::max(1, std::min(4096, (int)(ext[1] / w_target)));
+  g.w_x = ext[0] / g.ncx;
+  g.w_y = ext[1] / g.ncy;
+  if (!periodic) {  // make sure the top edge falls inside the last column
+    g.w_x *= 1.0 + 1e-12;
+    g.w_y *= 1.0 + 1e-12;
+  }
+  ncx = g.ncx;
+  ncy = g.ncy;
+  ncol = ncx * ncy;
+  max_clusters = n / kClusterSize + ncol + 1;
+
+  wx.resize(n); wy.resize(n); wz.resize(n);
+  atom_col.resize(n);
+  col_count.resize(ncol); col_fill.resize(ncol); col_atom_start.resize(ncol);
+  col_cl_start.resize(ncol); col_ncl.resize(ncol);
+  col_atoms.resize(n);
+  slot_orig.resize((size_t)max_clusters * kClusterSize);
+  orig_slot.resize(n);
+  n_clusters_d.resize(1);
+  cl_col.resize(max_clusters);
+  cl_lo.resize((size_t)max_clusters * 3);
+  cl_hi.resize((size_t)max_clusters * 3);
+  cl_entry_count.resize(max_clusters);
+  entry_counter.resize(2);
+
+  hipLaunchKernelGGL(k_fill_int, dim3((ncol + T - 1) / T), dim3(T), 0, stream, col_count.p, 0, ncol);
+  hipLaunchKernelGGL(k_bin, dim3(nb_atoms), dim3(T), 0, stream, n, g, x.p, y.p, z.p, wx.p, wy.p, wz.p,
+                     atom_col.p, col_count.p);
+  hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, stream, ncol, col_count.p, col_atom_start.p,
+                     col_cl_start.p, col_ncl.p, col_fill.p, n_clusters_d.p);
+  hipLaunchKernelGGL(k_fill_columns, dim3(nb_atoms), dim3(T), 0, stream, n, atom_col.p, col_atom_start.p,
+                     col_fill.p, col_atoms.p);
+  const int nslots = max_clusters * kClusterSize;
+  hipLaunchKernelGGL(k_fill_int, dim3((nslots + T - 1) / T), dim3(T), 0, stream, slot_orig.p, -1, nslots);
+  hipLaunchKernelGGL(k_sort_columns, dim3(nb_atoms), dim3(T), 0, stream, n, atom_col.p, col_count.p,
+                     col_atom_start.p, col_cl_start.p, col_atoms.p, wz.p, slot_orig.p, orig_slot.p, cl_col.p);
+  hipLaunchKernelGGL(k_cluster_bbox, dim3((max_clusters + T - 1) / T), dim3(T), 0, stream, n_clusters_d.p,
+                     max_clusters, slot_orig.p, wx.p, wy.p, wz.p, cl_lo.p, cl_hi.p);
+
+  // 3. pair search
+  SearchArgs A{};
+  A.periodic = g.periodic;
+  A.ncx = ncx;
+  A.ncy = ncy;
+  const double R = rlist + 1e-6 * std::max(1.0, rlist);
+  A.Dx = (int)(R / g.w_x) + 1;
+  A.Dy = (int)(R / g.w_y) + 1;
+  // Unwrapped column offsets in [-D, D] visit each (column, image) at most once;
+  // images beyond +-1 are skipped inside the kernel.
+  for (int d = 0; d < 3; d++) {
+    A.L[d] = periodic ? box[d] : 0.0;
+    A.half[d] = periodic ? box[d] / 2.0 : 0.0;
+  }
+  A.rlist2 = rlist * rlist;  // sqr(Cutoff + Padding), neighbourlist.cpp:355
+  A.Rbox2 = R * R;
+  A.mid = mid;
+  if (entry_capacity == 0) {
+    // half of the atoms within rlist of an 8-atom cluster's box, with headroom
+    double a = w_target;
+    double V = a * a * a + 6 * a * a * R + 3 * Consts::PI * a * R * R + 4.0 / 3.0 * Consts::PI * R * R * R;
+    double est = 0.5 * rho * V;
+    int stride = (int)(est * 1.6) + 64;
+    stride = (stride + 63) / 64 * 64;
+    entry_capacity = (size_t)stride;
+  }
+  for (int attempt = 0; attempt < 4; attempt++) {
+    A.stride = (int)entry_capacity;
+    entries.resize((size_t)max_clusters * entry_capacity);
+    PDGPU_HIP(hipMemsetAsync(entry_counter.p, 0, 2 * sizeof(unsigned long long), stream));
+    hipLaunchKernelGGL(k_pair_search, dim3(max_clusters), dim3(64), 0, stream, A, n_clusters_d.p, cl_col.p,
+                       col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p, slot_orig.p, x.p, y.p, z.p, wx.p, wy.p,
+                       wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p, entries.p, cl_entry_count.p,
+                       entry_counter.p);
+    PDGPU_HIP(hipGetLastError());
+    unsigned long long hc[2];
+    entry_counter.download(hc, 2, stream);
+    n_clusters_d.download(&n_clusters, 1, stream);
+    sync();
+    if (hc[1] == 0) break;
+    if (attempt == 3) throw EngineError(PDGPU_ERR_SETUP, "neighbour list does not fit (too many neighbours)");
+    entry_capacity = std::max(entry_capacity * 2, (size_t)(hc[1] * 1.25 + 64) / 64 * 64);
+  }
+  full_update_count++;
+  list_valid = true;
+}
+
+// PD's packed per-atom form of the pair set (neighbourlist.h:131-151), built on
+// the host from the device entries; used for parity checks and by callers that
+// read the list itself.
+void Engine::export_pairs(std::vector<int> &counts, std::vector<uint32_t> &packed) {
+  require(list_valid, "no neighbour list built");
+  std::vector<int> h_cnt(n_clusters), h_slot((size_t)n_clusters * kClusterSize);
+  std::vector<uint2> h_ent((size_t)n_clusters * entry_capacity);
+  cl_entry_count.download(h_cnt.data(), n_clusters, stream);
+  slot_orig.download(h_slot.data(), h_slot.size(), stream);
+  entries.download(h_ent.data(), h_ent.size(), stream);
+  sync();
+  std::vector<std::vector<uint32_t>> per(n);
+  for (int ci = 0; ci < n_clusters; ci++) {
+    for (int k = 0; k < h_cnt[ci]; k++) {
+      uint2 e = h_ent[(size_t)ci * entry_capacity + k];
+      int sb = (int)(e.x & 0xFFFFFFu);
+      int img = (int)(e.x >> 24);
+      int ob = h_slot[sb];
+      unsigned lm = e.y & 0xFFu, codes = e.y >> 8;
+      for (int ii = 0; ii < kClusterSize; ii++) {
+        if (!((lm >> ii) & 1u)) continue;
+        int oa = h_slot[ci * kClusterSize + ii];
+        int c = (codes >> (2 * ii)) & 3;
+        uint32_t bo = c == 0 ? 7u : (uint32_t)c;
+        int hi = std::max(oa, ob), lo = std::min(oa, ob);
+        int img_pd = (oa > ob) ? img : 26 - img;  // image of p_lo - p_hi
+        uint32_t w_hi = (uint32_t)lo | (bo << 24);
+        uint32_t w_lo = (uint32_t)hi | (bo << 24);
+        if (periodic) {
+          w_hi |= (uint32_t)(img_pd & 31) << 27;
+          w_lo |= (uint32_t)((26 - img_pd) & 31) << 27;
+        }
+        per[hi].push_back(w_hi);
+        if (calc_shadow) per[lo].push_back(w_lo);
+      }
+    }
+  }
+  counts.assign(n, 0);
+  packed.clear();
+  for (int i = 0; i < n; i++) {
+    auto &v = per[i];
+    std::sort(v.begin(), v.end(), [](uint32_t a, uint32_t b) { return (a & 0xFFFFFFu) < (b & 0xFFFFFFu); });
+    counts[i] = (int)v.size();
+    packed.insert(packed.end(), v.begin(), v.end());
+  }
+}
+
+}  // namespace pdgpu

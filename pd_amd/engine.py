@@ -1,0 +1,332 @@
+"""ctypes binding of the C ABI in include/pdgpu.h (pd_amd/libpdgpu.so).
+
+The library is the product: there is no CPU fallback.  Loading fails with
+ImportError when libpdgpu.so has not been built, and every call that returns a
+non-zero status raises PdGpuError carrying the engine's last error text.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdgpu.so")
+
+PDGPU_NONBONDED = 0x1
+PDGPU_BONDED = 0x2
+PDGPU_GB = 0x4
+PDGPU_VELOCITY_VERLET = 1
+PDGPU_LANGEVIN = 3
+
+ERRORS = {-1: "argument", -2: "hip", -3: "setup", -4: "nomem", -5: "unstable"}
+
+ENERGY_FIELDS = ("epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross",
+                 "epot_bond", "epot_angle", "epot_torsion", "epot_surf", "virial", "ekin")
+
+# Every entry point include/pdgpu.h declares.
+EXPORTS = (
+    "pdgpu_create", "pdgpu_destroy", "pdgpu_last_error", "pdgpu_version", "pdgpu_set_atom_params",
+    "pdgpu_set_molecules", "pdgpu_set_bond_order", "pdgpu_set_bond_order_from_cov", "pdgpu_set_space",
+    "pdgpu_request_cutoff", "pdgpu_set_padding", "pdgpu_calc_shadow", "pdgpu_setup_nonbonded",
+    "pdgpu_set_bonded", "pdgpu_setup_gb", "pdgpu_set_positions", "pdgpu_get_positions",
+    "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_build_list", "pdgpu_full_update_count",
+    "pdgpu_get_pair_count", "pdgpu_get_pairs", "pdgpu_compute", "pdgpu_get_born_radii", "pdgpu_md",
+)
+
+
+class PdGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdgpu error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Energies(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ENERGY_FIELDS]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in ENERGY_FIELDS}
+
+
+class NBParams(C.Structure):
+    """FF_NonBonded public parameters (nonbonded.h:72-143)."""
+    _fields_ = [("Cutoff", C.c_double), ("InnerCutoff", C.c_double), ("VdwCutoff", C.c_double),
+                ("VdwInnerCutoff", C.c_double), ("Vdw14Scaling", C.c_double),
+                ("Elec14Scaling", C.c_double), ("Dielectric", C.c_double),
+                ("DDDielectricAlpha", C.c_double), ("DoVdw", C.c_int32), ("DoElec", C.c_int32),
+                ("ForceSwitch", C.c_int32), ("EnergySwitch", C.c_int32), ("VdwCor", C.c_int32),
+                ("VdwCorDensity", C.c_double), ("VdwCorEpsilon", C.c_double),
+                ("VdwCorRadius", C.c_double), ("VdwCorCutoff", C.c_double)]
+
+
+class GBParams(C.Structure):
+    """FF_GeneralizedBorn_Still parameters (gbff.cpp:34-55)."""
+    _fields_ = [("Cutoff", C.c_double), ("InnerCutoff", C.c_double), ("VdwCutoff", C.c_double),
+                ("VdwInnerCutoff", C.c_double), ("Vdw14Scaling", C.c_double),
+                ("Elec14Scaling", C.c_double), ("DielectricSolute", C.c_double),
+                ("DielectricSolvent", C.c_double), ("GbsaStillCutoff", C.c_double),
+                ("ExpApproxThreshold", C.c_double), ("DielectricOffset", C.c_double),
+                ("BornRadiusOffset", C.c_double), ("P1", C.c_double), ("P2", C.c_double),
+                ("P3", C.c_double), ("P4", C.c_double), ("P5", C.c_double)]
+
+
+class MDParams(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("components", C.c_uint32), ("timestep", C.c_double),
+                ("temperature", C.c_double), ("fric_coeff", C.c_double), ("update_nlist", C.c_int32),
+                ("step_offset", C.c_int32), ("total_steps", C.c_int32), ("seed", C.c_uint64)]
+
+
+def nb_params(**kw) -> NBParams:
+    """Defaults of FF_NonBonded::settodefault (nonbonded.cpp:985-1021)."""
+    p = NBParams(Cutoff=15.0, InnerCutoff=12.0, VdwCutoff=15.0, VdwInnerCutoff=12.0,
+                 Vdw14Scaling=1.0, Elec14Scaling=1.0, Dielectric=1.0, DDDielectricAlpha=1.0,
+                 DoVdw=1, DoElec=1, ForceSwitch=1, EnergySwitch=0, VdwCor=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def gb_params(**kw) -> GBParams:
+    """Defaults of FF_GeneralizedBorn_Still (gbff.cpp:34-55) over FF_NonBonded's cutoffs."""
+    p = GBParams(Cutoff=15.0, InnerCutoff=12.0, VdwCutoff=15.0, VdwInnerCutoff=12.0,
+                 Vdw14Scaling=1.0, Elec14Scaling=1.0, DielectricSolute=1.0, DielectricSolvent=80.0,
+                 GbsaStillCutoff=10.0, ExpApproxThreshold=-4.0, DielectricOffset=-0.090,
+                 BornRadiusOffset=0.0, P1=0.073, P2=0.921, P3=6.211, P4=15.236, P5=1.254)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+_lib = None
+
+
+def load():
+    """Loads libpdgpu.so (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.pdgpu_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.pdgpu_destroy.argtypes = [vp]
+    L.pdgpu_last_error.argtypes = [vp]
+    L.pdgpu_last_error.restype = C.c_char_p
+    L.pdgpu_version.restype = C.c_char_p
+    L.pdgpu_set_atom_params.argtypes = [vp, dp, dp, dp, dp]
+    L.pdgpu_set_molecules.argtypes = [vp, C.c_int32, ip, ip]
+    L.pdgpu_set_bond_order.argtypes = [vp, C.c_int32, C.POINTER(C.c_int8), C.c_int32, ip, ip, ip]
+    L.pdgpu_set_bond_order_from_cov.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
+    L.pdgpu_set_space.argtypes = [vp, C.c_int32, dp]
+    L.pdgpu_request_cutoff.argtypes = [vp, C.c_double]
+    L.pdgpu_set_padding.argtypes = [vp, C.c_double]
+    L.pdgpu_calc_shadow.argtypes = [vp, C.c_int32]
+    L.pdgpu_setup_nonbonded.argtypes = [vp, C.POINTER(NBParams)]
+    L.pdgpu_set_bonded.argtypes = [vp, C.c_int32, ip, ip, dp, dp, C.c_int32, ip, ip, ip, dp, dp,
+                                   C.c_int32, ip, ip, ip, ip, ip, dp, dp, dp]
+    L.pdgpu_setup_gb.argtypes = [vp, C.POINTER(GBParams), dp, ip, ip, dp, ip, ip, dp]
+    for f in ("pdgpu_set_positions", "pdgpu_get_positions", "pdgpu_set_velocities", "pdgpu_get_velocities"):
+        getattr(L, f).argtypes = [vp, dp, C.c_size_t]
+    L.pdgpu_build_list.argtypes = [vp]
+    L.pdgpu_full_update_count.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.pdgpu_get_pair_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pdgpu_get_pairs.argtypes = [vp, ip, C.POINTER(C.c_uint32), C.c_int64]
+    L.pdgpu_compute.argtypes = [vp, C.c_uint32, dp, C.c_size_t, C.POINTER(Energies)]
+    L.pdgpu_get_born_radii.argtypes = [vp, dp]
+    L.pdgpu_md.argtypes = [vp, C.POINTER(MDParams), C.c_int32, dp, dp]
+    _lib = L
+    return L
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Engine:
+    """One system resident on one GPU (the pdgpu_engine handle)."""
+
+    def __init__(self, n_atoms: int, device: int = 0):
+        self.L = load()
+        self.n = int(n_atoms)
+        h = C.c_void_p()
+        rc = self.L.pdgpu_create(self.n, int(device), C.byref(h))
+        if rc != 0:
+            raise PdGpuError(rc, "pdgpu_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pdgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PdGpuError(rc, self.L.pdgpu_last_error(self.h).decode())
+
+    # ---- setup --------------------------------------------------------------
+    def set_atom_params(self, radius, epsilon, charge, mass):
+        a = [_d(v) for v in (radius, epsilon, charge, mass)]
+        self._chk(self.L.pdgpu_set_atom_params(self.h, *[p for _, p in a]))
+
+    def set_molecules(self, first, last):
+        f, fp = _i(first)
+        l, lp = _i(last)
+        self._chk(self.L.pdgpu_set_molecules(self.h, len(f), fp, lp))
+
+    def set_bond_order(self, mid, band, sp_i, sp_j, sp_order):
+        b = np.ascontiguousarray(band, dtype=np.int8)
+        si, sip = _i(sp_i if len(sp_i) else [0])
+        sj, sjp = _i(sp_j if len(sp_j) else [0])
+        so, sop = _i(sp_order if len(sp_order) else [0])
+        self._chk(self.L.pdgpu_set_bond_order(self.h, int(mid), b.ctypes.data_as(C.POINTER(C.c_int8)),
+                                              len(sp_i), sip, sjp, sop))
+
+    def set_bond_order_from_cov(self, resid, cov):
+        args = [_i(resid)] + [_i(cov[k] if len(cov[k]) else [0]) for k in
+                              ("off12", "idx12", "off13", "idx13", "off14", "idx14")]
+        self._chk(self.L.pdgpu_set_bond_order_from_cov(self.h, *[p for _, p in args]))
+
+    def set_space(self, box: Optional[np.ndarray]):
+        if box is None:
+            self._chk(self.L.pdgpu_set_space(self.h, 0, None))
+        else:
+            b, bp = _d(box)
+            self._chk(self.L.pdgpu_set_space(self.h, 1, bp))
+
+    def request_cutoff(self, c):
+        self._chk(self.L.pdgpu_request_cutoff(self.h, float(c)))
+
+    def set_padding(self, p):
+        self._chk(self.L.pdgpu_set_padding(self.h, float(p)))
+
+    def calc_shadow(self, on: bool):
+        self._chk(self.L.pdgpu_calc_shadow(self.h, int(bool(on))))
+
+    def setup_nonbonded(self, p: NBParams):
+        self._chk(self.L.pdgpu_setup_nonbonded(self.h, C.byref(p)))
+
+    def setup_gb(self, p: GBParams, gbrad, cov):
+        keep = [_d(gbrad)]
+        for k, conv in (("off12", _i), ("idx12", _i), ("d12", _d), ("off13", _i), ("idx13", _i), ("d13", _d)):
+            v = cov[k]
+            if len(v) == 0:
+                v = [0.0] if conv is _d else [0]
+            keep.append(conv(v))
+        self._chk(self.L.pdgpu_setup_gb(self.h, C.byref(p), *[q for _, q in keep]))
+
+    def set_bonded(self, s):
+        """FF_Bonded terms of a MolSystem (bonds, angles, torsions)."""
+        nb, na, nt = len(s.bonds), len(s.angles), len(s.torsions)
+        zi, zd = [0], [0.0, 0.0, 0.0, 0.0]
+        keep = []
+
+        def I(a):
+            keep.append(_i(a))
+            return keep[-1][1]
+
+        def D(a):
+            keep.append(_d(a))
+            return keep[-1][1]
+        args = [nb, I(s.bonds[:, 0] if nb else zi), I(s.bonds[:, 1] if nb else zi),
+                D(s.bond_l if nb else zd), D(s.bond_k if nb else zd),
+                na, I(s.angles[:, 0] if na else zi), I(s.angles[:, 1] if na else zi),
+                I(s.angles[:, 2] if na else zi), D(s.angle_t0 if na else zd), D(s.angle_k if na else zd),
+                nt] + [I(s.torsions[:, k] if nt else zi) for k in range(4)] + [
+                I(s.tor_terms if nt else zi),
+                D(np.asarray(s.tor_vn).reshape(-1) if nt else zd),
+                D(np.asarray(s.tor_n).reshape(-1) if nt else zd),
+                D(np.asarray(s.tor_gamma).reshape(-1) if nt else zd)]
+        self._chk(self.L.pdgpu_set_bonded(self.h, *args))
+
+    # ---- state --------------------------------------------------------------
+    def set_positions(self, pos):
+        p, pp = _d(np.asarray(pos).reshape(-1, 3))
+        self._chk(self.L.pdgpu_set_positions(self.h, pp, 24))
+
+    def get_positions(self):
+        out = np.zeros((self.n, 3))
+        self._chk(self.L.pdgpu_get_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), 24))
+        return out
+
+    def set_velocities(self, vel):
+        p, pp = _d(np.asarray(vel).reshape(-1, 3))
+        self._chk(self.L.pdgpu_set_velocities(self.h, pp, 24))
+
+    def get_velocities(self):
+        out = np.zeros((self.n, 3))
+        self._chk(self.L.pdgpu_get_velocities(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), 24))
+        return out
+
+    # ---- list ---------------------------------------------------------------
+    def build_list(self):
+        self._chk(self.L.pdgpu_build_list(self.h))
+
+    def full_update_count(self) -> int:
+        c = C.c_uint32()
+        self._chk(self.L.pdgpu_full_update_count(self.h, C.byref(c)))
+        return c.value
+
+    def get_pairs(self):
+        tot = C.c_int64()
+        self._chk(self.L.pdgpu_get_pair_count(self.h, C.byref(tot)))
+        counts = np.zeros(self.n, np.int32)
+        packed = np.zeros(max(tot.value, 1), np.uint32)
+        self._chk(self.L.pdgpu_get_pairs(self.h, counts.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         packed.ctypes.data_as(C.POINTER(C.c_uint32)), len(packed)))
+        return counts, packed[:tot.value]
+
+    # ---- evaluation ---------------------------------------------------------
+    def compute(self, components: int, forces: bool = True):
+        f = np.zeros((self.n, 3))
+        e = Energies()
+        fp = f.ctypes.data_as(C.POINTER(C.c_double)) if forces else None
+        self._chk(self.L.pdgpu_compute(self.h, int(components), fp, 24, C.byref(e)))
+        return (f if forces else None), e.as_dict()
+
+    def born_radii(self):
+        out = np.zeros(self.n)
+        self._chk(self.L.pdgpu_get_born_radii(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def md(self, params: MDParams, n_steps: int):
+        ep = np.zeros(max(n_steps, 1))
+        ek = np.zeros(max(n_steps, 1))
+        self._chk(self.L.pdgpu_md(self.h, C.byref(params), int(n_steps),
+                                  ep.ctypes.data_as(C.POINTER(C.c_double)),
+                                  ek.ctypes.data_as(C.POINTER(C.c_double))))
+        return ep[:n_steps], ek[:n_steps]
+
+
+def engine_for(sys, nb: Optional[NBParams] = None, padding: float = 0.5, calc_shadow: bool = False,
+               bonded: bool = False, device: int = 0) -> Engine:
+    """An Engine loaded with a MolSystem (pd_amd.systems) and an FF_NonBonded setup."""
+    e = Engine(sys.n, device)
+    e.set_atom_params(sys.radius, sys.epsilon, sys.charge, sys.mass)
+    e.set_molecules(sys.mol_first, sys.mol_last)
+    e.set_bond_order_from_cov(sys.resid, sys.cov)
+    e.set_space(sys.box)
+    e.set_padding(padding)
+    e.calc_shadow(calc_shadow)
+    if nb is not None:
+        e.setup_nonbonded(nb)
+    if bonded:
+        e.set_bonded(sys)
+    e.set_positions(sys.pos)
+    return e

@@ -1,0 +1,557 @@
+"""Builders for the synthetic inputs of the benchmark workloads.
+
+Everything is in PD's units (positions in Angstrom, masses in kg, per-atom
+epsilon as sqrt(J), force constants in J/A^2 / J/rad^2 / J, angles in rad).
+
+* ``water_box``: N TIP3P waters on the even lattice of
+  PeriodicBox::getEvenPointDistribution (space.cpp:126-186), each rotated by a
+  uniform random rotation about its centre of geometry, as System::solvate_N
+  does (system.cpp:137-231).  TIP3P parameters are those of param/tip3.ff.
+* ``peptide_chain``: a synthetic protein-like chain.  PD's residue templates and
+  polymer builder (ffparam.cpp, genpolymer.cpp) are not part of the engine, so
+  chains are generated here from 20 residue recipes with AMBER99 atom-type
+  parameters, backbone phi/psi drawn from N(-60,10)/N(-45,10) degrees and
+  bonded terms whose equilibrium values are the built geometry.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import units as U
+
+
+@dataclass
+class MolSystem:
+    """One molecular system in PD units, plus its covalent topology."""
+
+    name: str
+    pos: np.ndarray              # (n,3) float64, Angstrom
+    radius: np.ndarray           # R_min/2, Angstrom
+    epsilon: np.ndarray          # sqrt(eps[J])
+    charge: np.ndarray           # e
+    mass: np.ndarray             # kg
+    gbrad: np.ndarray            # GB (Still) radius of the atom type, Angstrom
+    resid: np.ndarray            # residue index per atom
+    mol_first: np.ndarray        # first atom of each molecule
+    mol_last: np.ndarray         # last atom of each molecule (inclusive)
+    bonds: np.ndarray            # (nb,2) int32
+    bond_l: np.ndarray           # Angstrom
+    bond_k: np.ndarray           # J/A^2
+    angles: np.ndarray           # (na,3) int32: i, a(apex), j
+    angle_t0: np.ndarray         # rad
+    angle_k: np.ndarray          # J/rad^2
+    torsions: np.ndarray         # (nt,4) int32: i, a, b, j
+    tor_terms: np.ndarray        # (nt,) int32
+    tor_vn: np.ndarray           # (nt,4) J
+    tor_n: np.ndarray            # (nt,4)
+    tor_gamma: np.ndarray        # (nt,4) rad
+    box: Optional[np.ndarray] = None   # (3,) for a periodic box, None for vacuum
+    cov: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    def copy(self) -> "MolSystem":
+        import copy as _c
+        return _c.deepcopy(self)
+
+
+# ---------------------------------------------------------------------------
+# covalent neighbour lists (PD's Particle::cov12atom/cov13atom/cov14atom)
+# ---------------------------------------------------------------------------
+
+def _csr(lists):
+    off = np.zeros(len(lists) + 1, dtype=np.int32)
+    for i, l in enumerate(lists):
+        off[i + 1] = off[i] + len(l)
+    idx = np.array([j for l in lists for j in l], dtype=np.int32)
+    return off, idx
+
+
+def build_covalent(n: int, bonds: np.ndarray, bond_l: np.ndarray, angles: np.ndarray,
+                   angle_t0: np.ndarray) -> dict:
+    """1-2, 1-3 and 1-4 partner lists with the equilibrium distances GB needs.
+
+    d12 is the bond's equilibrium length (ffbonded.cpp:149); d13 follows the
+    cosine rule on the angle's equilibrium values (ffbonded.cpp:303).
+    """
+    nb12: List[list] = [[] for _ in range(n)]
+    d12: List[list] = [[] for _ in range(n)]
+    for (i, j), l in zip(bonds.tolist(), bond_l.tolist()):
+        nb12[i].append(j); d12[i].append(l)
+        nb12[j].append(i); d12[j].append(l)
+    bl = {}
+    for (i, j), l in zip(bonds.tolist(), bond_l.tolist()):
+        bl[(i, j)] = l
+        bl[(j, i)] = l
+    nb13: List[list] = [[] for _ in range(n)]
+    d13: List[list] = [[] for _ in range(n)]
+    for (i, a, j), t0 in zip(angles.tolist(), angle_t0.tolist()):
+        db1 = bl[(i, a)]
+        db2 = bl[(a, j)]
+        d = math.sqrt(db1 * db1 + db2 * db2 - 2 * db1 * db2 * math.cos(t0))
+        for (x, y) in ((i, j), (j, i)):
+            if y not in nb13[x] and y not in nb12[x]:
+                nb13[x].append(y); d13[x].append(d)
+    nb14: List[list] = [[] for _ in range(n)]
+    for a in range(n):
+        for b in nb12[a]:
+            for c in nb12[b]:
+                if c == a:
+                    continue
+                for d in nb12[c]:
+                    if d in (a, b) or d in nb12[a] or d in nb13[a] or d in nb14[a]:
+                        continue
+                    nb14[a].append(d)
+    off12, idx12 = _csr(nb12)
+    off13, idx13 = _csr(nb13)
+    off14, idx14 = _csr(nb14)
+    return dict(off12=off12, idx12=idx12, d12=np.array([x for l in d12 for x in l], dtype=np.float64),
+                off13=off13, idx13=idx13, d13=np.array([x for l in d13 for x in l], dtype=np.float64),
+                off14=off14, idx14=idx14)
+
+
+# ---------------------------------------------------------------------------
+# random rotations (Shoemake, "Uniform random rotations", Graphics Gems III)
+# ---------------------------------------------------------------------------
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    u1, u2, u3 = rng.random(3)
+    a = math.sqrt(1.0 - u1)
+    b = math.sqrt(u1)
+    qx, qy, qz, qw = (a * math.sin(U.TWO_PI * u2), a * math.cos(U.TWO_PI * u2),
+                      b * math.sin(U.TWO_PI * u3), b * math.cos(U.TWO_PI * u3))
+    return np.array([
+        [1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)],
+        [2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)],
+        [2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)],
+    ])
+
+
+# ---------------------------------------------------------------------------
+# water box
+# ---------------------------------------------------------------------------
+
+TIP3P_OH = 0.9572                  # param/tip3.ff BOND OW HW
+TIP3P_HOH = 104.52                 # param/tip3.ff ANGLE HW OW HW (degrees)
+TIP3P_K_BOND = 553.0               # kcal/mol/A^2
+TIP3P_K_ANGLE = 100.0              # kcal/mol/rad^2
+TIP3P_Q = (-0.834, 0.417, 0.417)
+TIP3P_MASS = (16.00, 1.01, 1.01)   # amu
+TIP3P_R = (1.7682, 0.0, 0.0)
+TIP3P_EPS = (0.1521, 0.0, 0.0)     # kcal/mol
+
+
+def even_points(box: np.ndarray, n: int) -> np.ndarray:
+    """PeriodicBox::getEvenPointDistribution (space.cpp:126-186)."""
+    bx, by, bz = (float(v) for v in box)
+    dnx = (bx * bx / (by * bz) * n) ** (1.0 / 3.0)
+    dny = (by * by / (bx * bz) * n) ** (1.0 / 3.0)
+    dnz = (bz * bz / (bx * by) * n) ** (1.0 / 3.0)
+    inx, iny, inz = math.ceil(dnx), math.ceil(dny), math.ceil(dnz)
+    best = (inx, iny, inz)
+    bestx = inx * iny * inz - n
+    if bestx < 0:
+        bestx = 1 << 62
+    for i in range(inx - 2, inx + 5):
+        for j in range(iny - 2, iny + 5):
+            for k in range(inz - 2, inz + 5):
+                ex = i * j * k - n
+                if 0 <= ex < bestx:
+                    bestx = ex
+                    best = (i, j, k)
+    nx, ny, nz = best
+    step = np.array([bx / nx, by / ny, bz / nz])
+    half = np.array([bx, by, bz]) / 2.0
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pts = np.stack([step[0] * (ii.ravel() + 0.5), step[1] * (jj.ravel() + 0.5),
+                    step[2] * (kk.ravel() + 0.5)], axis=1) - half
+    return pts[:n]
+
+
+def water_box(n_waters: int = 32768, seed: int = 2024, density: float = 1.0,
+              box_len: Optional[float] = None) -> MolSystem:
+    """TIP3P box at `density` g/cm^3 (cubic), orientations uniform-random from `seed`."""
+    rng = np.random.default_rng(seed)
+    if box_len is None:
+        grams = n_waters * sum(TIP3P_MASS) * U.AMU * 1e3
+        box_len = (grams / density) ** (1.0 / 3.0) * 1e8  # cm -> A
+    box = np.array([box_len] * 3, dtype=np.float64)
+    pts = even_points(box, n_waters)
+    h = math.radians(TIP3P_HOH) / 2.0
+    tmpl = np.array([[0.0, 0.0, 0.0],
+                     [TIP3P_OH * math.sin(h), TIP3P_OH * math.cos(h), 0.0],
+                     [-TIP3P_OH * math.sin(h), TIP3P_OH * math.cos(h), 0.0]])
+    tmpl -= tmpl.mean(axis=0)
+    pos = np.empty((3 * n_waters, 3), dtype=np.float64)
+    for w in range(n_waters):
+        R = random_rotation(rng)
+        pos[3 * w:3 * w + 3] = tmpl @ R.T + pts[w]
+    n = 3 * n_waters
+    rep = lambda t: np.tile(np.array(t, dtype=np.float64), n_waters)
+    bonds = np.stack([np.repeat(np.arange(n_waters) * 3, 2),
+                      (np.arange(n_waters)[:, None] * 3 + np.array([1, 2])).ravel()], axis=1).astype(np.int32)
+    nb = bonds.shape[0]
+    angles = np.stack([np.arange(n_waters) * 3 + 1, np.arange(n_waters) * 3,
+                       np.arange(n_waters) * 3 + 2], axis=1).astype(np.int32)
+    na = angles.shape[0]
+    s = MolSystem(
+        name=f"tip3p_{n_waters}",
+        pos=pos,
+        radius=rep(TIP3P_R),
+        epsilon=np.sqrt(rep(TIP3P_EPS) * U.KCAL2J / U.NA),
+        charge=rep(TIP3P_Q),
+        mass=rep(TIP3P_MASS) * U.AMU,
+        gbrad=rep((1.48, 1.15, 1.15)),
+        resid=np.repeat(np.arange(n_waters, dtype=np.int32), 3),
+        mol_first=(np.arange(n_waters) * 3).astype(np.int32),
+        mol_last=(np.arange(n_waters) * 3 + 2).astype(np.int32),
+        bonds=bonds,
+        bond_l=np.full(nb, TIP3P_OH),
+        bond_k=np.full(nb, U.kcal_to_j(TIP3P_K_BOND)),
+        angles=angles,
+        angle_t0=np.full(na, math.radians(TIP3P_HOH) * 0 + TIP3P_HOH * U.PI_OVER_180),
+        angle_k=np.full(na, U.kcal_to_j(TIP3P_K_ANGLE)),
+        torsions=np.zeros((0, 4), dtype=np.int32),
+        tor_terms=np.zeros(0, dtype=np.int32),
+        tor_vn=np.zeros((0, 4)), tor_n=np.zeros((0, 4)), tor_gamma=np.zeros((0, 4)),
+        box=box,
+    )
+    s.cov = build_covalent(n, s.bonds, s.bond_l, s.angles, s.angle_t0)
+    return s
+
+
+# ---------------------------------------------------------------------------
+# synthetic peptide chains
+# ---------------------------------------------------------------------------
+
+# AMBER99 atom types (param/amber99aa.ff TYPE lines and its GB section):
+#          R(A)    eps(kcal/mol) mass(amu) GB radius(A)
+ATYPES = {
+    "N":  (1.8240, 0.1700, 14.01, 1.625),
+    "N3": (1.8240, 0.1700, 14.01, 1.625),
+    "H":  (0.6000, 0.0157, 1.01, 1.15),
+    "CT": (1.9080, 0.1094, 12.01, 1.875),
+    "C":  (1.9080, 0.0860, 12.01, 1.875),
+    "CA": (1.9080, 0.0860, 12.01, 1.875),
+    "H1": (1.3870, 0.0157, 1.01, 1.15),
+    "HC": (1.4870, 0.0157, 1.01, 1.15),
+    "HA": (1.4590, 0.0150, 1.01, 1.15),
+    "O":  (1.6612, 0.2100, 16.00, 1.48),
+    "O2": (1.6612, 0.2100, 16.00, 1.48),
+    "OH": (1.7210, 0.2104, 16.00, 1.535),
+    "HO": (0.0000, 0.0000, 1.01, 1.15),
+    "S":  (2.0000, 0.2500, 32.06, 1.775),
+    "SH": (2.0000, 0.2500, 32.06, 1.775),
+    "HS": (0.6000, 0.0157, 1.01, 1.15),
+}
+# default partial charges per type; residues are neutralised on the side chain
+QDEF = {"N": -0.4157, "H": 0.2719, "CT": -0.10, "C": 0.60, "CA": -0.12, "H1": 0.08,
+        "HC": 0.05, "HA": 0.12, "O": -0.5679, "O2": -0.80, "OH": -0.65, "HO": 0.42,
+        "S": -0.27, "SH": -0.31, "HS": 0.19, "N3": -0.30}
+VALENCE = {"CT": 4, "C": 3, "CA": 3, "N": 3, "N3": 4, "O": 1, "O2": 1, "OH": 2, "S": 2, "SH": 2}
+HTYPE = {"CT": "HC", "CA": "HA", "N": "H", "N3": "H", "OH": "HO", "SH": "HS"}
+# side chains as (type, parent) with parent -1 = CA, net residue charge
+RESIDUES = {
+    "GLY": ([], 0), "ALA": ([("CT", -1)], 0),
+    "SER": ([("CT", -1), ("OH", 0)], 0), "CYS": ([("CT", -1), ("SH", 0)], 0),
+    "VAL": ([("CT", -1), ("CT", 0), ("CT", 0)], 0), "THR": ([("CT", -1), ("OH", 0), ("CT", 0)], 0),
+    "LEU": ([("CT", -1), ("CT", 0), ("CT", 1), ("CT", 1)], 0),
+    "ILE": ([("CT", -1), ("CT", 0), ("CT", 0), ("CT", 1)], 0),
+    "MET": ([("CT", -1), ("CT", 0), ("S", 1), ("CT", 2)], 0),
+    "PRO": ([("CT", -1), ("CT", 0), ("CT", 1)], 0),
+    "ASP": ([("CT", -1), ("C", 0), ("O2", 1), ("O2", 1)], -1),
+    "GLU": ([("CT", -1), ("CT", 0), ("C", 1), ("O2", 2), ("O2", 2)], -1),
+    "ASN": ([("CT", -1), ("C", 0), ("O", 1), ("N", 1)], 0),
+    "GLN": ([("CT", -1), ("CT", 0), ("C", 1), ("O", 2), ("N", 2)], 0),
+    "LYS": ([("CT", -1), ("CT", 0), ("CT", 1), ("CT", 2), ("N3", 3)], 1),
+    "ARG": ([("CT", -1), ("CT", 0), ("CT", 1), ("N", 2), ("CA", 3), ("N", 4), ("N", 4)], 1),
+    "HIS": ([("CT", -1), ("CA", 0), ("CA", 1), ("N", 2), ("CA", 3), ("N", 1)], 0),
+    "PHE": ([("CT", -1), ("CA", 0), ("CA", 1), ("CA", 2), ("CA", 3), ("CA", 4), ("CA", 5)], 0),
+    "TYR": ([("CT", -1), ("CA", 0), ("CA", 1), ("CA", 2), ("CA", 3), ("CA", 4), ("CA", 5), ("OH", 4)], 0),
+    "TRP": ([("CT", -1), ("CA", 0), ("CA", 1), ("N", 2), ("CA", 3), ("CA", 4), ("CA", 5), ("CA", 6),
+             ("CA", 7), ("CA", 8)], 0),
+}
+RES_NAMES = sorted(RESIDUES)
+BOND_LEN = {frozenset(("C", "C")): 1.525, frozenset(("C", "N")): 1.47, frozenset(("C", "O")): 1.25,
+            frozenset(("C", "S")): 1.81, frozenset(("C", "H")): 1.09, frozenset(("N", "H")): 1.01,
+            frozenset(("O", "H")): 0.96, frozenset(("S", "H")): 1.336}
+
+
+def _elem(t: str) -> str:
+    return t[0]
+
+
+def _place(a, b, c, bond, angle, torsion):
+    """NeRF: the point d with |cd|=bond, angle(b,c,d)=angle, dihedral(a,b,c,d)=torsion (rad)."""
+    bc = c - b
+    bc /= np.linalg.norm(bc)
+    nrm = np.cross(b - a, bc)
+    nn = np.linalg.norm(nrm)
+    if nn < 1e-8:
+        nrm = np.cross(bc, np.array([1.0, 0.3, 0.1]))
+        nn = np.linalg.norm(nrm)
+    nrm /= nn
+    m = np.stack([bc, np.cross(nrm, bc), nrm], axis=1)
+    d2 = np.array([-bond * math.cos(angle), bond * math.sin(angle) * math.cos(torsion),
+                   bond * math.sin(angle) * math.sin(torsion)])
+    return c + m @ d2
+
+
+def _fill_sites(pos_a, nbr_dirs, n_new, sp2, ref_dir):
+    """Unit directions for `n_new` substituents completing a tetrahedral (sp3) or
+    trigonal (sp2) centre whose existing bond directions are `nbr_dirs`."""
+    u = [d / np.linalg.norm(d) for d in nbr_dirs]
+    if len(u) == 1:
+        u1 = u[0]
+        e1 = ref_dir - (ref_dir @ u1) * u1
+        if np.linalg.norm(e1) < 1e-6:
+            e1 = np.cross(u1, np.array([0.3, 1.0, 0.2]))
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(u1, e1)
+        if sp2:
+            th = math.radians(120.0)
+            phis = [0.0, math.pi][:n_new]
+        else:
+            th = math.radians(109.5)
+            phis = [math.pi + k * 2.0 * math.pi / 3.0 for k in range(n_new)]
+        return [math.cos(th) * u1 + math.sin(th) * (math.cos(p) * e1 + math.sin(p) * e2) for p in phis]
+    if len(u) == 2:
+        b = -(u[0] + u[1]); b /= np.linalg.norm(b)
+        if sp2 or n_new == 1:
+            if sp2:
+                return [b]
+        nrm = np.cross(u[0], u[1]); nrm /= np.linalg.norm(nrm)
+        h = math.radians(54.75)
+        out = [math.cos(h) * b + math.sin(h) * nrm, math.cos(h) * b - math.sin(h) * nrm]
+        return out[:n_new]
+    b = -(u[0] + u[1] + u[2]); b /= np.linalg.norm(b)
+    return [b][:n_new]
+
+
+def _relieve_clashes(x, bonds, nbrs, is_h, iters=300):
+    """Position-based clash relief: push apart atoms that are not 1-2/1-3 partners
+    and closer than 2.0 A (pairs with a hydrogen) or 2.7 A (heavy pairs), while
+    projecting the 1-2 and 1-3 distances back to their built values."""
+    from scipy.spatial import cKDTree
+    n = x.shape[0]
+    c13 = set()
+    for a in range(n):
+        for i in nbrs[a]:
+            c13.add((min(a, i), max(a, i)))
+            for j in nbrs[a]:
+                if i < j:
+                    c13.add((i, j))
+    cons = np.array(sorted(c13), dtype=np.int64).reshape(-1, 2)
+    d0 = np.linalg.norm(x[cons[:, 0]] - x[cons[:, 1]], axis=1)
+    for _ in range(iters):
+        tree = cKDTree(x)
+        pr = tree.query_pairs(2.7, output_type="ndarray")
+        if len(pr):
+            key = pr[:, 0] * n + pr[:, 1]
+            ck = cons[:, 0] * n + cons[:, 1]
+            pr = pr[~np.isin(key, ck)]
+        moved = False
+        if len(pr):
+            lim = np.where(is_h[pr[:, 0]] | is_h[pr[:, 1]], 2.0, 2.7)
+            v = x[pr[:, 1]] - x[pr[:, 0]]
+            d = np.linalg.norm(v, axis=1)
+            bad = d < lim
+            if bad.any():
+                moved = True
+                v, d, lim, p = v[bad], d[bad], lim[bad], pr[bad]
+                s = (0.5 * (lim - d) / np.maximum(d, 1e-6))[:, None] * v
+                np.add.at(x, p[:, 0], -s)
+                np.add.at(x, p[:, 1], s)
+        for _k in range(3):
+            v = x[cons[:, 1]] - x[cons[:, 0]]
+            d = np.linalg.norm(v, axis=1)
+            s = (0.5 * (d - d0) / np.maximum(d, 1e-9))[:, None] * v
+            np.add.at(x, cons[:, 0], s * 0.5)
+            np.add.at(x, cons[:, 1], -s * 0.5)
+        if not moved:
+            break
+    return x
+
+
+def peptide_chain(n_res: int, seed: int = 7, phi_mu: float = -60.0, phi_sd: float = 10.0,
+                  psi_mu: float = -45.0, psi_sd: float = 10.0,
+                  sequence: Optional[List[str]] = None) -> MolSystem:
+    """Random-sequence chain of `n_res` residues in vacuum (InfiniteSpace)."""
+    rng = np.random.default_rng(seed)
+    if sequence is None:
+        sequence = [RES_NAMES[k] for k in rng.integers(0, len(RES_NAMES), n_res)]
+    types: List[str] = []
+    resid: List[int] = []
+    P = np.zeros((64 * max(1, len(sequence)), 3))
+    bonds: List[tuple] = []
+    nbrs: List[list] = []
+    res_q = []
+
+    def add(t, r, p):
+        types.append(t); resid.append(r); nbrs.append([])
+        P[len(types) - 1] = p
+        return len(types) - 1
+
+    def bond(i, j):
+        bonds.append((i, j)); nbrs[i].append(j); nbrs[j].append(i)
+
+    def clear_of(xyz, skip, lim=2.3):
+        k = len(types)
+        if k == 0:
+            return 1e9
+        d = np.linalg.norm(P[:k] - xyz, axis=1)
+        if skip:
+            d[list(skip)] = 1e9
+        return float(d.min())
+
+    tet = math.radians(109.5)
+    prev = None
+    for r, name in enumerate(sequence):
+        phi = math.radians(rng.normal(phi_mu, phi_sd))
+        psi = math.radians(rng.normal(psi_mu, psi_sd))
+        if prev is None:
+            n_at = add("N", r, [0.0, 0.0, 0.0])
+            ca = add("CT", r, [1.458, 0.0, 0.0])
+            c_at = add("C", r, _place(np.array([0.0, 1.0, 0.0]), P[n_at], P[ca], 1.525,
+                                       math.radians(111.2), math.radians(-60.0)))
+        else:
+            pn, pca, pc, ppsi = prev
+            n_at = add("N", r, _place(P[pn], P[pca], P[pc], 1.329, math.radians(116.2), ppsi))
+            ca = add("CT", r, _place(P[pca], P[pc], P[n_at], 1.458, math.radians(121.7), math.pi))
+            c_at = add("C", r, _place(P[pc], P[n_at], P[ca], 1.525, math.radians(111.2), phi))
+            bond(pc, n_at)
+        bond(n_at, ca)
+        bond(ca, c_at)
+        # amide H: bisector of the C(prev)-N-CA angle, pointing away
+        dirs = [P[x] - P[n_at] for x in nbrs[n_at]]
+        hdir = _fill_sites(P[n_at], dirs, 1, True, P[c_at] - P[n_at])[0] if len(dirs) == 2 else \
+            _fill_sites(P[n_at], dirs, 1, False, np.array([0.0, 1.0, 0.0]))[0]
+        h = add("H", r, P[n_at] + 1.01 * hdir)
+        bond(n_at, h)
+        sc, net = RESIDUES[name]
+        # CA substituents: CB (or second HA for GLY) and HA in the two free tetrahedral sites
+        dirs = [P[n_at] - P[ca], P[c_at] - P[ca]]
+        sites = _fill_sites(P[ca], dirs, 2, False, None)
+        ha = add("H1", r, P[ca] + 1.09 * sites[1])
+        bond(ca, ha)
+        sc_idx: List[int] = []
+        parent_of = {ca: n_at}
+        if not sc:
+            ha2 = add("H1", r, P[ca] + 1.09 * sites[0])
+            bond(ca, ha2)
+        for k, (t, par) in enumerate(sc):
+            p_at = ca if par < 0 else sc_idx[par]
+            if par < 0:
+                xyz = P[ca] + 1.53 * sites[0]
+            else:
+                gp = parent_of[p_at]
+                ggp = parent_of.get(gp, c_at)
+                bl = BOND_LEN.get(frozenset((_elem(types[p_at]), _elem(t))), 1.5)
+                ang = math.radians(120.0 if types[p_at] in ("C", "CA") else 111.0)
+                best, bestd = None, -1.0
+                skip = {p_at, gp}
+                for attempt in range(12):
+                    tor = math.radians(float(rng.choice([60.0, 180.0, -60.0])) + rng.normal(0, 10.0)
+                                       + 30.0 * attempt)
+                    cand = _place(P[ggp], P[gp], P[p_at], bl, ang, tor)
+                    dmin = clear_of(cand, skip)
+                    if dmin > bestd:
+                        best, bestd = cand, dmin
+                    if dmin >= 2.6:
+                        break
+                xyz = best
+            a = add(t, r, xyz)
+            bond(p_at, a)
+            parent_of[a] = p_at
+            sc_idx.append(a)
+        o = add("O", r, _place(P[n_at], P[ca], P[c_at], 1.229, math.radians(120.5), psi + math.pi))
+        bond(c_at, o)
+        # hydrogens on side-chain heavy atoms, filling the free valence sites
+        for a in sc_idx:
+            t = types[a]
+            nh = VALENCE.get(t, 0) - len(nbrs[a])
+            if t in ("C", "O", "O2", "S") or nh <= 0:
+                continue
+            dirs = [P[x] - P[a] for x in nbrs[a]]
+            par = nbrs[a][0]
+            ref = P[[x for x in nbrs[par] if x != a][0]] - P[par]
+            sp2 = t in ("CA", "N")
+            bl = BOND_LEN[frozenset((_elem(t), "H"))]
+            for d in _fill_sites(P[a], dirs, nh, sp2, ref):
+                hx = add(HTYPE[t], r, P[a] + bl * d)
+                bond(a, hx)
+        prev = (n_at, ca, c_at, psi)
+        first = [i for i in range(len(types)) if resid[i] == r]
+        q = sum(QDEF[types[i]] for i in first)
+        res_q.append((net - q, sc_idx if sc_idx else [ca]))
+    n = len(types)
+    charge = np.array([QDEF[t] for t in types], dtype=np.float64)
+    for dq, targets in res_q:
+        charge[targets] += dq / len(targets)
+    prm = np.array([ATYPES[t] for t in types], dtype=np.float64)
+    bonds_a = np.array(bonds, dtype=np.int32)
+    posa = _relieve_clashes(P[:n].copy(), bonds_a, nbrs, np.array([t[0] == "H" for t in types]))
+    posa -= posa.mean(axis=0)
+    bond_l = np.linalg.norm(posa[bonds_a[:, 0]] - posa[bonds_a[:, 1]], axis=1)
+    bond_k = np.array([U.kcal_to_j(340.0 if "H" in (types[i][0], types[j][0]) else 317.0)
+                       for i, j in bonds_a.tolist()])
+    angles, t0, ak = [], [], []
+    for a in range(n):
+        for x in range(len(nbrs[a])):
+            for y in range(x + 1, len(nbrs[a])):
+                i, j = nbrs[a][x], nbrs[a][y]
+                u = posa[i] - posa[a]; v = posa[j] - posa[a]
+                th = math.acos(max(-1.0, min(1.0, float(u @ v / (np.linalg.norm(u) * np.linalg.norm(v))))))
+                angles.append((i, a, j)); t0.append(th)
+                ak.append(U.kcal_to_j(35.0 if "H" in (types[i][0], types[j][0]) else 63.0))
+    tors = []
+    for (a, b) in bonds_a.tolist():
+        for i in nbrs[a]:
+            if i == b:
+                continue
+            for j in nbrs[b]:
+                if j == a or j == i:
+                    continue
+                tors.append((i, a, b, j))
+    nt = len(tors)
+    tor_terms = np.ones(nt, dtype=np.int32)
+    tor_vn = np.zeros((nt, 4)); tor_vn[:, 0] = U.kcal_to_j(0.15)
+    tor_n = np.zeros((nt, 4)); tor_n[:, 0] = 3.0
+    tor_gamma = np.zeros((nt, 4))
+    s = MolSystem(
+        name=f"peptide_{len(sequence)}",
+        pos=posa, radius=prm[:, 0].copy(), epsilon=np.sqrt(prm[:, 1] * U.KCAL2J / U.NA),
+        charge=charge, mass=prm[:, 2] * U.AMU, gbrad=prm[:, 3].copy(),
+        resid=np.array(resid, dtype=np.int32),
+        mol_first=np.array([0], dtype=np.int32), mol_last=np.array([n - 1], dtype=np.int32),
+        bonds=bonds_a, bond_l=bond_l, bond_k=bond_k,
+        angles=np.array(angles, dtype=np.int32).reshape(-1, 3), angle_t0=np.array(t0), angle_k=np.array(ak),
+        torsions=np.array(tors, dtype=np.int32).reshape(-1, 4), tor_terms=tor_terms,
+        tor_vn=tor_vn, tor_n=tor_n, tor_gamma=tor_gamma, box=None,
+    )
+    s.cov = build_covalent(n, s.bonds, s.bond_l, s.angles, s.angle_t0)
+    return s
+
+
+def min_nonbonded_distance(s: MolSystem) -> float:
+    """Smallest distance between atoms that are neither 1-2 nor 1-3 partners."""
+    from scipy.spatial import cKDTree
+    tree = cKDTree(s.pos)
+    pairs = tree.query_pairs(2.5, output_type="ndarray")
+    if len(pairs) == 0:
+        return 2.5
+    excl = set()
+    c = s.cov
+    for i in range(s.n):
+        for k in range(c["off12"][i], c["off12"][i + 1]):
+            excl.add((i, int(c["idx12"][k])))
+        for k in range(c["off13"][i], c["off13"][i + 1]):
+            excl.add((i, int(c["idx13"][k])))
+    d = [np.linalg.norm(s.pos[i] - s.pos[j]) for i, j in pairs.tolist() if (i, j) not in excl]
+    return float(min(d)) if d else 2.5
