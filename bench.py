@@ -1,0 +1,213 @@
+"""Benchmark: ns/day of NVE MD on the 98,304-atom TIP3P box (one MI355X).
+
+Workload (the explicit-solvent configuration): 32,768 TIP3P waters on the even
+lattice with uniformly random orientations (seeded), cubic box 99.346 A at
+1.000 g/cm^3, orthogonal PBC, LJ + Coulomb cutoff 10 A with 8-10 A switching
+(Coulomb force switch, LJ energy switch), list padding 2 A rebuilt every 10
+steps, flexible TIP3P bonds/angle, PD's velocity Verlet at 1 fs.  A "step" is
+one MD step: the list refresh when due (cleanSpace + rebuild), nonbonded +
+bonded forces and the FP64 integrator, all on the device.
+
+Before timing, the lattice is relaxed by device Langevin dynamics (untimed) so
+the NVE window starts from a physical state; the warmup steps are NVE.
+
+Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector/matrix peak (dense)
+# Algorithmic FLOPs per unit of work, from PD's expression trees (add/sub/mul = 1,
+# FMA = 2, sqrt/rcp/div = 1): a list entry tested and outside the cutoff, a pair
+# at r <= InnerCutoff (LJ + force-switched Coulomb inner branch + accumulation),
+# a pair in the switching shell (adds the LJ switch and the Coulomb polynomial).
+FLOP_TESTED, FLOP_INNER, FLOP_SHELL = 11, 53, 81
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pair_classes(eng, box, cutoff, inner):
+    """Counts of PD list entries by distance class at the current positions."""
+    counts, packed = eng.get_pairs()
+    pos = eng.get_positions()
+    i = np.repeat(np.arange(len(counts)), counts)
+    j = (packed & 0xFFFFFF).astype(np.int64)
+    bo = (packed >> 24) & 7
+    img = (packed >> 27).astype(np.int64)
+    sh = np.stack([(img // 9) - 1, ((img // 3) % 3) - 1, (img % 3) - 1], axis=1) * box[None, :]
+    n_tot = len(packed)
+    in_cut = inner_cnt = 0
+    step = 4_000_000
+    for a in range(0, n_tot, step):
+        b = min(n_tot, a + step)
+        d = pos[j[a:b]] - pos[i[a:b]] + sh[a:b]
+        r2 = (d * d).sum(axis=1)
+        keep = (bo[a:b] >= 3)
+        in_cut += int(((r2 <= cutoff * cutoff) & keep).sum())
+        inner_cnt += int(((r2 <= inner * inner) & keep).sum())
+    return n_tot, in_cut, inner_cnt
+
+
+def cpu_baseline(sysw, nb_kw, padding):
+    """The FP64 oracle (PD's algorithms restated in C, one thread) on the same box:
+    one O(N^2) list build, one nonbonded + bonded evaluation and one VV step."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    pos = O.clean_space(sysw)
+    bo = O.bond_order(sysw)
+    p = O.nb_params(**nb_kw)
+    cut = max(p.Cutoff, p.VdwCutoff)
+    counts, packed = O.build_list(sysw, pos, cut, padding, False, bo)
+    t_list = time.perf_counter() - t0
+    basis = O.basis_vectors(True, sysw.box, cut)
+    t1 = time.perf_counter()
+    f, _ = O.nb_forces(sysw, pos, counts, packed, p, basis)
+    fb, _ = O.bonded_forces(sysw, pos)
+    vel = np.zeros_like(pos)
+    O.vv_step(sysw.mass, 1e-15, f + fb, pos, vel)
+    t_step = time.perf_counter() - t1
+    per_step = t_list / 10.0 + t_step
+    return {
+        "value": round(86400.0 / per_step * 1e-6, 6),  # 1 fs per step -> ns/day
+        "unit": "ns/day",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (PD's O(N^2) list build + fast2 nonbonded + bonded + velocity Verlet, "
+                   f"FP64, 1 thread) on the same 98,304-atom box: 1 list build {t_list:.2f} s "
+                   f"(amortised over 10 steps) + 1 force/integrator step {t_step:.2f} s"),
+        "s_per_step": per_step,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--relax", type=int, default=300, help="untimed Langevin relaxation steps")
+    ap.add_argument("--waters", type=int, default=32768)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pd_amd import engine as G
+    from pd_amd import systems as S
+    from pd_amd import units as U
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    t0 = time.perf_counter()
+    w = S.water_box(args.waters, seed=2024)
+    nb_kw = dict(Cutoff=10.0, InnerCutoff=8.0, VdwCutoff=10.0, VdwInnerCutoff=8.0, ForceSwitch=1)
+    padding = 2.0
+    eng = G.engine_for(w, G.nb_params(**nb_kw), padding=padding, bonded=True)
+    comps = G.PDGPU_NONBONDED | G.PDGPU_BONDED
+    log(f"built {w.n} atoms, box {w.box[0]:.3f} A in {time.perf_counter() - t0:.1f} s")
+
+    # untimed relaxation of the lattice (Langevin, 0.5 fs, strong friction)
+    relax = G.MDParams(integrator=G.PDGPU_LANGEVIN, components=comps, timestep=0.5e-15, temperature=300.0,
+                       fric_coeff=5e13, update_nlist=10, step_offset=0, total_steps=1 << 30, seed=11)
+    if args.relax:
+        eng.md(relax, args.relax)
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(w.n, 3)) * np.sqrt(U.KB * 300.0 / w.mass)[:, None]
+    eng.set_velocities(v)
+    log(f"relaxed {args.relax} steps")
+
+    nve = G.MDParams(integrator=G.PDGPU_VELOCITY_VERLET, components=comps, timestep=1e-15, temperature=300.0,
+                     fric_coeff=0.0, update_nlist=10, step_offset=0, total_steps=args.warmup + args.steps, seed=1)
+    eng.md(nve, args.warmup)
+    # pair classes at the start of the timed window -> algorithmic FLOPs per nonbonded launch
+    eng.build_list()
+    n_tot, n_cut, n_inner = pair_classes(eng, w.box, nb_kw["Cutoff"], nb_kw["InnerCutoff"])
+    flops = (n_tot - n_cut) * FLOP_TESTED + n_inner * FLOP_INNER + (n_cut - n_inner) * FLOP_SHELL
+    log(f"list entries {n_tot}, in cutoff {n_cut}, inner {n_inner}; {flops / 1e9:.3f} GFLOP per nonbonded launch")
+
+    nve.step_offset = args.warmup
+    eng.set_timing(True)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    ep, ek = eng.md(nve, args.steps)
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    nb_ms, nb_launch = eng.kernel_time("nonbonded")
+    ps_ms, ps_launch = eng.kernel_time("pair_search")
+    eng.set_timing(False)
+
+    wall = te - ts
+    ms_per_step = wall * 1e3 / args.steps
+    ns_day = args.steps * 1.0 * 1e-6 * 86400.0 / wall
+    nb_avg_s = nb_ms / max(nb_launch, 1) / 1e3
+    achieved = flops / nb_avg_s / 1e12
+    etot = (np.asarray(ep) + np.asarray(ek)) * U.J_TO_KCALMOL
+    drift = float(np.polyfit(np.arange(len(etot)), etot, 1)[0]) * 1e6 if len(etot) > 2 else 0.0  # per ns at 1 fs
+
+    out = {
+        "metric": "ns/day @98k-atom water box; batched GB/SA steps/s vs plain, at stated error",
+        "value": round(ns_day, 3),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "tip3p_water_box_98304_nve",
+            "n_atoms": w.n,
+            "box_A": round(float(w.box[0]), 4),
+            "cutoff_A": 10.0, "switch_A": [8.0, 10.0], "padding_A": padding, "update_nlist": 10,
+            "timestep_fs": 1.0, "integrator": "velocity_verlet (PD form), FP64",
+            "pair_math": "FP32, FP64 displacement/cutoff test, int64 fixed-point forces",
+            "parallelism": "single GPU",
+        },
+        "pair_interactions_per_s": round(n_cut * args.steps / wall, 1),
+        "nve_drift_kcal_mol_per_ns": round(drift, 3),
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_nonbonded<force switch, vdw>",
+            "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+            "traffic": None,
+            "avg_launch_us": round(nb_avg_s * 1e6, 3),
+            "launches": nb_launch,
+            "algorithmic_flop_per_launch": int(flops),
+            "list_entries": int(n_tot),
+            "pairs_in_cutoff": int(n_cut),
+        },
+        "list_build": {
+            "kernel": "k_pair_search",
+            "avg_launch_us": round(ps_ms / max(ps_launch, 1) * 1e3, 3),
+            "launches": ps_launch,
+            "algorithmic_bytes": int(w.n * 24 + n_tot * 4),
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline (oracle, 1 thread)...")
+        cb = cpu_baseline(w, nb_kw, padding)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -209,6 +209,16 @@ int pdgpu_get_born_radii(pdgpu_engine *e, double *radii);
 int pdgpu_md(pdgpu_engine *e, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
              double *ekin_trace);
 
+/* ---- instrumentation ---------------------------------------------------------------- */
+
+/* Kernel timing (no counterpart in PD, whose MD timer has 1 s resolution,
+ * md.cpp:194-196).  While enabled, the engine brackets launches of its named
+ * kernels ("nonbonded", "pair_search", "gb_radii", "gb_pair", "gb_chain") with HIP
+ * events on its own stream.  Enabling resets the totals. */
+int pdgpu_set_timing(pdgpu_engine *e, int32_t enable);
+/* Summed duration (ms) and launch count of one named kernel since timing was enabled. */
+int pdgpu_kernel_time(pdgpu_engine *e, const char *kernel, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

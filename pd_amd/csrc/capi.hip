@@ -25,7 +25,40 @@ Engine::Engine(int natoms, int dev) : n(natoms), device(dev) {
 }
 
 Engine::~Engine() {
+  (void)hipStreamSynchronize(stream);
+  for (auto &kv : timers)
+    for (auto &pr : kv.second.pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (hipEvent_t ev : event_pool) (void)hipEventDestroy(ev);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+hipEvent_t Engine::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t ev = event_pool.back();
+    event_pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev;
+  PDGPU_HIP(hipEventCreate(&ev));
+  return ev;
+}
+
+void Engine::resolve_timers() {
+  sync();
+  for (auto &kv : timers) {
+    for (auto &pr : kv.second.pending) {
+      float ms = 0.0f;
+      PDGPU_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      kv.second.ms += ms;
+      kv.second.launches++;
+      event_pool.push_back(pr.first);
+      event_pool.push_back(pr.second);
+    }
+    kv.second.pending.clear();
+  }
 }
 
 namespace {
@@ -410,6 +443,27 @@ int pdgpu_compute(pdgpu_engine *h, uint32_t components, double *f_accum, size_t 
     if (energies) *energies = tmp;
     if (std::isnan(tmp.epot))
       throw EngineError(PDGPU_ERR_UNSTABLE, "Simulation unstable or division by zero");
+  });
+}
+
+int pdgpu_set_timing(pdgpu_engine *h, int32_t enable) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.resolve_timers();
+    E.timing = enable != 0;
+    for (auto &kv : E.timers) {
+      kv.second.ms = 0.0;
+      kv.second.launches = 0;
+    }
+  });
+}
+
+int pdgpu_kernel_time(pdgpu_engine *h, const char *kernel, double *total_ms, int64_t *launches) {
+  return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(kernel && total_ms && launches, "null argument");
+    E.resolve_timers();
+    auto it = E.timers.find(kernel);
+    *total_ms = it == E.timers.end() ? 0.0 : it->second.ms;
+    *launches = it == E.timers.end() ? 0 : it->second.launches;
   });
 }
 

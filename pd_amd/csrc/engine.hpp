@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -190,7 +191,19 @@ struct Engine {
   // list.hip
   void build_list();
   void export_pairs(std::vector<int> &counts, std::vector<uint32_t> &packed);
+  // kernel timing instrumentation (HIP events on `stream`)
+  bool timing = false;
+  struct TimerSlot {
+    double ms = 0.0;
+    long long launches = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  };
+  std::map<std::string, TimerSlot> timers;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  void resolve_timers();
   // nonbonded.hip
+  void prepare_slots();
   void setup_nonbonded(const pdgpu_nb_params &p);
   void gather_positions();
   void launch_nonbonded(double *energy_slots);  // energies: vdw, elec, virial
@@ -208,6 +221,29 @@ struct Engine {
   bool have_gb = false;
   void launch_gb(double *energy_slots);
   void gb_energies_to_host(const double *slots, pdgpu_energies &e);
+};
+
+// Brackets the launches in its scope with HIP events on the engine's stream
+// when timing is on (pdgpu_set_timing).
+struct ScopedTimer {
+  Engine &E;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ScopedTimer(Engine &e, const char *n) : E(e), name(n) {
+    if (E.timing) {
+      a = E.take_event();
+      (void)hipEventRecord(a, E.stream);
+    }
+  }
+  ~ScopedTimer() {
+    if (E.timing) {
+      b = E.take_event();
+      (void)hipEventRecord(b, E.stream);
+      auto &slot = E.timers[name];
+      slot.pending.emplace_back(a, b);
+      if (slot.pending.size() > 4096) E.resolve_timers();
+    }
+  }
 };
 
 void langevin_coefficients(double gammat, double *c6);

@@ -522,10 +522,13 @@ void Engine::build_list() {
     A.stride = (int)entry_capacity;
     entries.resize((size_t)max_clusters * entry_capacity);
     PDGPU_HIP(hipMemsetAsync(entry_counter.p, 0, 2 * sizeof(unsigned long long), stream));
-    hipLaunchKernelGGL(k_pair_search, dim3(max_clusters), dim3(64), 0, stream, A, n_clusters_d.p, cl_col.p,
-                       col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p, slot_orig.p, x.p, y.p, z.p, wx.p, wy.p,
-                       wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p, entries.p, cl_entry_count.p,
-                       entry_counter.p);
+    {
+      ScopedTimer t(*this, "pair_search");
+      hipLaunchKernelGGL(k_pair_search, dim3(max_clusters), dim3(64), 0, stream, A, n_clusters_d.p, cl_col.p,
+                         col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p, slot_orig.p, x.p, y.p, z.p, wx.p, wy.p,
+                         wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p, entries.p, cl_entry_count.p,
+                         entry_counter.p);
+    }
     PDGPU_HIP(hipGetLastError());
     unsigned long long hc[2];
     entry_counter.download(hc, 2, stream);
@@ -536,6 +539,7 @@ void Engine::build_list() {
     entry_capacity = std::max(entry_capacity * 2, (size_t)(hc[1] * 1.25 + 64) / 64 * 64);
   }
   full_update_count++;
+  prepare_slots();
   list_valid = true;
 }
 

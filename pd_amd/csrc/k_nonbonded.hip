@@ -335,9 +335,8 @@ void Engine::gather_positions() {
                      slot_orig.p, x.p, y.p, z.p, sx.p, sy.p, sz.p);
 }
 
-void Engine::launch_nonbonded(double *energy_out) {
-  require(have_nb, "nonbonded component not set up");
-  require(list_valid, "neighbour list not built");
+// Per-slot parameters and accumulators; called once per list rebuild.
+void Engine::prepare_slots() {
   const int nslots = max_clusters * kClusterSize;
   sq.resize(nslots);
   sr.resize(nslots);
@@ -345,18 +344,28 @@ void Engine::launch_nonbonded(double *energy_out) {
   nbx.resize(nslots);
   nby.resize(nslots);
   nbz.resize(nslots);
-  hipLaunchKernelGGL(k_gather_params, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots, n_clusters_d.p,
-                     slot_orig.p, q_scaled.p, lj_r.p, lj_e.p, sq.p, sr.p, se.p);
+  if (have_nb)
+    hipLaunchKernelGGL(k_gather_params, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots,
+                       n_clusters_d.p, slot_orig.p, q_scaled.p, lj_r.p, lj_e.p, sq.p, sr.p, se.p);
+}
+
+void Engine::launch_nonbonded(double *energy_out) {
+  require(have_nb, "nonbonded component not set up");
+  require(list_valid, "neighbour list not built");
+  const int nslots = max_clusters * kClusterSize;
   hipLaunchKernelGGL(k_zero_ll, dim3((nslots + 255) / 256), dim3(256), 0, stream, nbx.p, nby.p, nbz.p, nslots);
   gather_positions();
   const int grid = (max_clusters + 7) / 8 * 8;
   e_partial.resize((size_t)grid * 4);
   const bool v = nbk.vdw_on != 0;
-  switch (nbk.elec_mode) {
-    case 0: v ? launch<EL_NONE, true>(*this, grid) : launch<EL_NONE, false>(*this, grid); break;
-    case 1: v ? launch<EL_PLAIN, true>(*this, grid) : launch<EL_PLAIN, false>(*this, grid); break;
-    case 3: v ? launch<EL_ESWITCH, true>(*this, grid) : launch<EL_ESWITCH, false>(*this, grid); break;
-    default: v ? launch<EL_FSWITCH, true>(*this, grid) : launch<EL_FSWITCH, false>(*this, grid); break;
+  {
+    ScopedTimer t(*this, "nonbonded");
+    switch (nbk.elec_mode) {
+      case 0: v ? launch<EL_NONE, true>(*this, grid) : launch<EL_NONE, false>(*this, grid); break;
+      case 1: v ? launch<EL_PLAIN, true>(*this, grid) : launch<EL_PLAIN, false>(*this, grid); break;
+      case 3: v ? launch<EL_ESWITCH, true>(*this, grid) : launch<EL_ESWITCH, false>(*this, grid); break;
+      default: v ? launch<EL_FSWITCH, true>(*this, grid) : launch<EL_FSWITCH, false>(*this, grid); break;
+    }
   }
   PDGPU_HIP(hipGetLastError());
   reduce4(stream, e_partial.p, grid, energy_out);

@@ -34,6 +34,7 @@ EXPORTS = (
     "pdgpu_set_bonded", "pdgpu_setup_gb", "pdgpu_set_positions", "pdgpu_get_positions",
     "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_build_list", "pdgpu_full_update_count",
     "pdgpu_get_pair_count", "pdgpu_get_pairs", "pdgpu_compute", "pdgpu_get_born_radii", "pdgpu_md",
+    "pdgpu_set_timing", "pdgpu_kernel_time",
 )
 
 
@@ -139,6 +140,8 @@ def load():
     L.pdgpu_compute.argtypes = [vp, C.c_uint32, dp, C.c_size_t, C.POINTER(Energies)]
     L.pdgpu_get_born_radii.argtypes = [vp, dp]
     L.pdgpu_md.argtypes = [vp, C.POINTER(MDParams), C.c_int32, dp, dp]
+    L.pdgpu_set_timing.argtypes = [vp, C.c_int32]
+    L.pdgpu_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -304,6 +307,16 @@ class Engine:
         out = np.zeros(self.n)
         self._chk(self.L.pdgpu_get_born_radii(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def set_timing(self, on: bool):
+        self._chk(self.L.pdgpu_set_timing(self.h, int(bool(on))))
+
+    def kernel_time(self, name: str):
+        """(total ms, launches) of a named kernel since timing was enabled."""
+        ms = C.c_double()
+        k = C.c_int64()
+        self._chk(self.L.pdgpu_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(k)))
+        return ms.value, k.value
 
     def md(self, params: MDParams, n_steps: int):
         ep = np.zeros(max(n_steps, 1))
