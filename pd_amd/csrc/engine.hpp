@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -115,6 +116,64 @@ struct NBKernelParams {
   int elec_mode;      // 0 none, 1 plain, 3 energy switch, 4 force switch
   double vdwcor_per_atom;  // kcal/mol added per atom when VdwCor
 };
+
+// ---- Generalized Born (k_gb.hip) ------------------------------------------------
+// Kernel constants of FF_GeneralizedBorn_Still in kcal/mol and A.
+struct GBK {
+  double cut2;          // Cutoff^2
+  double still2;        // GbsaStillCutoff^2
+  double vdwcut2;       // VdwCutoff^2
+  float premul;         // -halfeconv (1/eps_in - 1/eps_out), kcal/mol A
+  float ec_in, ec_out;  // econv / eps_in, econv / eps_out
+  float inner, invw, invw2_4;
+  float vdwinner, vinvw, vinvw2_4;
+  float thresh;         // ExpApproxThreshold
+  float P4, P5, invP5;
+  float vdw14, elec14;
+  float pi;
+};
+// One 64x64 tile pair of one system: first atoms of the i and j tiles (global
+// indices) and whether bond orders must be looked up.
+struct GBWork {
+  int sys, i0, j0, special;
+};
+struct GBSys {
+  int first, count;     // atom range in the concatenated arrays
+  int mid;              // BondOrder::m_MaxIndexDelta
+  long band_off;        // offset of the system's band matrix in the concatenated array
+  int sp_base;          // offset of the system's sparse CSR offsets (count+1 entries)
+};
+
+struct Engine;
+
+// Device arrays of the GB passes for one or more systems laid end to end.
+struct GBState {
+  pdgpu_gb_params prm{};
+  GBK kp{};
+  int n_atoms_total = 0;
+  std::vector<GBWork> h_work;
+  std::vector<GBSys> h_sys;
+  std::vector<int8_t> h_band;
+  std::vector<int> h_sp_off, h_sp_part, h_sp_ord, h_work_beg, h_work_end;
+  DBuf<GBWork> work;
+  DBuf<GBSys> sysd;
+  DBuf<int8_t> band;
+  DBuf<int> sp_off, sp_part, sp_ord, work_beg, work_end, sys_first, sys_count;
+  DBuf<long long> gpol, deda;
+  DBuf<double> born, eself, e_part, terms123;
+  DBuf<float> bornf, wgt, q, ljr, lje, gbrad, Vj;
+  ~GBState();
+  void set_params(const pdgpu_gb_params &p);
+  void add_system(int first, int count, int mid, const std::vector<int8_t> &band_sys, const std::vector<int> &spi,
+                  const std::vector<int> &spj, const std::vector<int> &spo);
+  void upload(hipStream_t s);
+  void run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
+           long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner);
+};
+
+void gb_fixed_terms(int n, const double *gbrad, const pdgpu_gb_params &p, const int32_t *off12,
+                    const int32_t *idx12, const double *d12, const int32_t *off13, const int32_t *idx13,
+                    const double *d13, std::vector<double> &Vj, std::vector<double> &terms123);
 
 struct Engine {
   int n = 0;
@@ -219,6 +278,7 @@ struct Engine {
   uint32_t last_components = 0;
   // k_gb.hip
   bool have_gb = false;
+  std::unique_ptr<GBState> gb;
   void launch_gb(double *energy_slots);
   void gb_energies_to_host(const double *slots, pdgpu_energies &e);
 };
@@ -242,6 +302,26 @@ struct ScopedTimer {
       auto &slot = E.timers[name];
       slot.pending.emplace_back(a, b);
       if (slot.pending.size() > 4096) E.resolve_timers();
+    }
+  }
+};
+
+// ScopedTimer for code paths that may run without an owning Engine.
+struct ScopedTimerOpt {
+  Engine *E;
+  const char *name;
+  hipEvent_t a = nullptr;
+  ScopedTimerOpt(Engine *e, const char *n) : E(e), name(n) {
+    if (E && E->timing) {
+      a = E->take_event();
+      (void)hipEventRecord(a, E->stream);
+    }
+  }
+  ~ScopedTimerOpt() {
+    if (E && E->timing) {
+      hipEvent_t b = E->take_event();
+      (void)hipEventRecord(b, E->stream);
+      E->timers[name].pending.emplace_back(a, b);
     }
   }
 };

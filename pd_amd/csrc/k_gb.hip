@@ -1,19 +1,697 @@
-// k_gb.hip -- FF_GeneralizedBorn_Still on the device (placeholder until the GB passes land).
+// k_gb.hip -- FF_GeneralizedBorn_Still (FastMode) on the device, for one system
+// or a batch of independent systems.
+//
+// PD evaluates three passes over its (shadow) neighbour list each step:
+//   1. calcBornRadii_PairwiseApprox (gbff.cpp:432-483): Gpol_i = terms123_i +
+//      sum_j P4 Vj CCF / r^4 over non-bonded/1-4 partners within GbsaStillCutoff,
+//      a_i = -halfeconv / Gpol_i + BornRadiusOffset;
+//   2. the pair pass of calcForcesIncludingVacuo (gbff.cpp:1546-1746): the Still
+//      cross term premul q_i q_j / sqrt(r^2 + a_i a_j exp(-r^2/4 a_i a_j)) with its
+//      dE/dr and dE/d(a_i a_j), vacuum Coulomb (eps_solvent where the GB term is
+//      dropped, uGB <= ExpApproxThreshold), LJ, energy switching, plus the self
+//      terms premul q^2 / a (gbff.cpp:1750-1757);
+//   3. the Born-radius chain rule (gbff.cpp:1768-1825) with gpi = a_i^2 / 166.0.
+// Here every pass is an all-pairs walk over 64x64 atom tiles of each system
+// (upper triangle of tile pairs), one wavefront per tile pair: lanes hold the i
+// atoms, the j tile sits in LDS and is visited in the rotated order
+// j = (lane + t) mod 64, so every lane touches a different j in each step and
+// the j accumulators in LDS need no atomics.  Pairs are tested with PD's
+// per-pair cutoffs (FP64 r^2 as PD computes it); the radii, dE/da and force
+// sums are 64-bit fixed point, so they are independent of summation order.
+// Tile pairs whose atoms lie within BondOrder::m_MaxIndexDelta of each other,
+// or hold an off-diagonal covalent partner, are flagged and look the bond order
+// up exactly as the list builder does; all other pairs have order 7.
+#include <algorithm>
+#include <cmath>
+
 #include "engine.hpp"
 
 namespace pdgpu {
 
-void Engine::launch_gb(double *) {
-  throw EngineError(PDGPU_ERR_SETUP, "GB component not available in this build");
+namespace {
+
+constexpr int kTile = 64;
+
+// Bond order of pair (a, b) as NeighbourList::calcNewList + correctOffDiagonalBondOrders
+// assign it (neighbourlist.cpp:399-406, 179-220); indices are local to the system.
+__device__ __forceinline__ int gb_bond_order(int a, int b, int mid, const int8_t *__restrict__ band,
+                                             const int *__restrict__ sp_off, const int *__restrict__ sp_part,
+                                             const int *__restrict__ sp_ord) {
+  int hi = max(a, b), lo = min(a, b);
+  int bo = 7;
+  if (hi - lo < mid) bo = band[(long)lo * mid + (hi - lo - 1)];
+  for (int k = sp_off[lo]; k < sp_off[lo + 1]; k++)
+    if (sp_part[k] == hi) bo = sp_ord[k];
+  return bo & 7;
 }
 
-void Engine::gb_energies_to_host(const double *, pdgpu_energies &) {}
+__device__ __forceinline__ bool tile_pair_valid(int lane, int t, int ni, int nj, bool diag, int jj) {
+  bool v = lane < ni && jj < nj;
+  if (diag) v = v && ((t >= 1 && t < 32) || (t == 32 && lane < 32));
+  return v;
+}
+
+// ---- pass 1: Born radii -------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict__ work, const GBSys *__restrict__ sys,
+                                                 const double *__restrict__ x, const double *__restrict__ y,
+                                                 const double *__restrict__ z, const float *__restrict__ gbrad,
+                                                 const float *__restrict__ Vj, const int8_t *__restrict__ band,
+                                                 const int *__restrict__ sp_off, const int *__restrict__ sp_part,
+                                                 const int *__restrict__ sp_ord, long long *__restrict__ gpol) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sR[kTile], sV[kTile], acc[kTile];
+  const GBWork w = work[blockIdx.x];
+  const GBSys S = sys[w.sys];
+  const int lane = threadIdx.x;
+  const int end = S.first + S.count;
+  const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
+  const bool diag = w.i0 == w.j0;
+  {
+    const int j = w.j0 + lane;
+    const bool v = lane < nj;
+    sx[lane] = v ? x[j] : 0.0;
+    sy[lane] = v ? y[j] : 0.0;
+    sz[lane] = v ? z[j] : 0.0;
+    sR[lane] = v ? gbrad[j] : 0.0f;
+    sV[lane] = v ? Vj[j] : 0.0f;
+    acc[lane] = 0.0f;
+  }
+  const int i = w.i0 + lane;
+  const bool iv = lane < ni;
+  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
+  const float Ri = iv ? gbrad[i] : 0.0f, Vi = iv ? Vj[i] : 0.0f;
+  __syncthreads();
+  float gi = 0.0f;
+  for (int t = 0; t < kTile; t++) {
+    const int jj = (lane + t) & (kTile - 1);
+    bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
+    double r2;
+    {
+#pragma clang fp contract(off)
+      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+      r2 = dx * dx + dy * dy + dz * dz;  // sqrdist, gbff.cpp:455
+    }
+    on = on && r2 <= K.still2;
+    if (w.special && on) {
+      const int bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off,
+                                   sp_off + S.sp_base, sp_part, sp_ord);
+      on = bo > 2;  // only non-bonded and 1-4 partners (gbff.cpp:460-461)
+    }
+    const float r2f = on ? (float)r2 : 1.0f;
+    const float Rs = Ri + sR[jj];
+    const float rdivR = r2f / (Rs * Rs);
+    float ccf = 1.0f;
+    if (rdivR <= K.invP5) {
+      const float c = 0.5f - 0.5f * cosf(rdivR * K.P5 * K.pi);
+      ccf = c * c;
+    }
+    const float base = K.P4 * ccf / (r2f * r2f);
+    gi += on ? base * sV[jj] : 0.0f;
+    acc[jj] += on ? base * Vi : 0.0f;
+  }
+  __syncthreads();
+  if (iv) fix_add(&gpol[i], fix_from_float(gi));
+  if (lane < nj) fix_add(&gpol[w.j0 + lane], fix_from_float(acc[lane]));
+}
+
+// a_i = -halfeconv / (terms123_i + sum) + BornRadiusOffset; the self term
+// premul q^2 / a seeds dE/da (gbff.cpp:1750-1757).
+__global__ void k_gb_finalize_radii(int n, double halfeconv, double offset, double premul,
+                                    const double *__restrict__ terms123, const long long *__restrict__ gpol,
+                                    const float *__restrict__ q, double *__restrict__ born,
+                                    float *__restrict__ bornf, long long *__restrict__ deda,
+                                    double *__restrict__ eself) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double G = terms123[i] + (double)gpol[i] * (1.0 / kFixScale);
+  const double a = -halfeconv / G + offset;
+  born[i] = a;
+  bornf[i] = (float)a;
+  const double qi = q[i];
+  const double epol = premul * qi * qi / a;
+  eself[i] = epol;
+  deda[i] = fix_from_double(-epol / a);
+}
+
+// ---- pass 2: GB cross terms, vacuum Coulomb and LJ ----------------------------------
+__global__ __launch_bounds__(64) void k_gb_pair(
+    GBK K, const GBWork *__restrict__ work, const GBSys *__restrict__ sys, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const float *__restrict__ q,
+    const float *__restrict__ ljr, const float *__restrict__ lje, const float *__restrict__ born,
+    const int8_t *__restrict__ band, const int *__restrict__ sp_off, const int *__restrict__ sp_part,
+    const int *__restrict__ sp_ord, long long *__restrict__ fx, long long *__restrict__ fy,
+    long long *__restrict__ fz, long long *__restrict__ deda, double *__restrict__ e_part) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sq[kTile], sr[kTile], se[kTile], sa[kTile];
+  __shared__ float ax[kTile], ay[kTile], az[kTile], ad[kTile];
+  const GBWork w = work[blockIdx.x];
+  const GBSys S = sys[w.sys];
+  const int lane = threadIdx.x;
+  const int end = S.first + S.count;
+  const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
+  const bool diag = w.i0 == w.j0;
+  {
+    const int j = w.j0 + lane;
+    const bool v = lane < nj;
+    sx[lane] = v ? x[j] : 0.0;
+    sy[lane] = v ? y[j] : 0.0;
+    sz[lane] = v ? z[j] : 0.0;
+    sq[lane] = v ? q[j] : 0.0f;
+    sr[lane] = v ? ljr[j] : 0.0f;
+    se[lane] = v ? lje[j] : 0.0f;
+    sa[lane] = v ? born[j] : 1.0f;
+    ax[lane] = ay[lane] = az[lane] = ad[lane] = 0.0f;
+  }
+  const int i = w.i0 + lane;
+  const bool iv = lane < ni;
+  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
+  const float qi = iv ? q[i] : 0.0f, ri = iv ? ljr[i] : 0.0f, ei = iv ? lje[i] : 0.0f;
+  const float ai = iv ? born[i] : 1.0f;
+  __syncthreads();
+  float fix = 0.0f, fiy = 0.0f, fiz = 0.0f, dai = 0.0f;
+  double epol = 0.0, evdw = 0.0, eel = 0.0;
+  for (int t = 0; t < kTile; t++) {
+    const int jj = (lane + t) & (kTile - 1);
+    bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
+    double dx, dy, dz, r2;
+    {
+#pragma clang fp contract(off)
+      dx = xi - sx[jj];
+      dy = yi - sy[jj];
+      dz = zi - sz[jj];
+      r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1588-1591
+    }
+    on = on && r2 <= K.cut2;
+    int bo = 7;
+    if (w.special && on)
+      bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off, sp_off + S.sp_base, sp_part,
+                         sp_ord);
+    const float r2f = on ? (float)r2 : 1.0f;
+    const float rinv = __builtin_amdgcn_rsqf(r2f);
+    const float r = r2f * rinv;
+    const float aj = sa[jj];
+    const float qq = qi * sq[jj];
+    const float aiaj = ai * aj;
+    const float uGB = -r2f * 0.25f / aiaj;
+    const bool nb = bo >= 3;                      // vdW/elec only for non-bonded and 1-4 pairs
+    const float s14e = bo == 3 ? K.elec14 : 1.0f;
+    // switching (r >= InnerCutoff), gbff.cpp:1640-1651
+    const float st = fmaxf(r - K.inner, 0.0f);
+    const float S0 = 1.0f - (K.invw * st) * (K.invw * st);
+    const float dS = -K.invw2_4 * st * S0;
+    const float Sw = S0 * S0;
+    float ep = 0.0f, dedr = 0.0f, dal = 0.0f, Eel = 0.0f, Fel = 0.0f;
+    if (uGB > K.thresh) {
+      const float tGB = __expf(uGB);
+      const float lGB = 1.0f / (r2f + tGB * aiaj);
+      const float fGB = sqrtf(lGB);
+      ep = K.premul * qq * fGB;
+      dedr = -ep * (r - 0.25f * r * tGB) * lGB;
+      dal = -ep * tGB * (1.0f - uGB) * lGB;
+      if (nb) {
+        Eel = K.ec_in * s14e * qq * rinv;
+        Fel = -Eel * rinv;
+      }
+      Fel = Sw * Fel + Eel * dS;
+      Eel *= Sw;
+      dedr = Sw * dedr + dS * ep;
+      dal *= Sw;
+      ep *= Sw;
+    } else if (nb) {
+      Eel = K.ec_out * s14e * qq * rinv;
+      Fel = -Eel * rinv;
+      Fel = Sw * Fel + Eel * dS;
+      Eel *= Sw;
+    }
+    float Ev = 0.0f, Fv = 0.0f;
+    if (nb && r2 < K.vdwcut2) {
+      const float eps = (bo == 3 ? K.vdw14 : 1.0f) * ei * se[jj];
+      const float Rij = ri + sr[jj];
+      const float s2 = Rij * Rij * rinv * rinv;
+      const float B = s2 * s2 * s2;
+      const float A = B * B;
+      Ev = eps * (A - 2.0f * B);
+      Fv = -12.0f * eps * rinv * (A - B);
+      const float vt = fmaxf(r - K.vdwinner, 0.0f);
+      const float V0 = 1.0f - (K.vinvw * vt) * (K.vinvw * vt);
+      const float vdS = -K.vinvw2_4 * vt * V0;
+      const float VS = V0 * V0;
+      Fv = VS * Fv + Ev * vdS;
+      Ev *= VS;
+    }
+    // gradient along d = p_i - p_j: GB part 2 dedr / r, vdW + elec (dE/dr) / r
+    const float g = on ? (2.0f * dedr + (nb ? (Fv + Fel) : 0.0f)) * rinv : 0.0f;
+    const float gx = g * (float)dx, gy = g * (float)dy, gz = g * (float)dz;
+    fix -= gx;
+    fiy -= gy;
+    fiz -= gz;
+    ax[jj] += gx;
+    ay[jj] += gy;
+    az[jj] += gz;
+    dai += on ? dal * aj : 0.0f;
+    ad[jj] += on ? dal * ai : 0.0f;
+    epol += on ? (double)ep : 0.0;
+    evdw += on && nb ? (double)Ev : 0.0;
+    eel += on && nb ? (double)Eel : 0.0;
+  }
+  __syncthreads();
+  if (iv) {
+    fix_add(&fx[i], fix_from_float(fix));
+    fix_add(&fy[i], fix_from_float(fiy));
+    fix_add(&fz[i], fix_from_float(fiz));
+    fix_add(&deda[i], fix_from_float(dai));
+  }
+  if (lane < nj) {
+    const int j = w.j0 + lane;
+    fix_add(&fx[j], fix_from_float(ax[lane]));
+    fix_add(&fy[j], fix_from_float(ay[lane]));
+    fix_add(&fz[j], fix_from_float(az[lane]));
+    fix_add(&deda[j], fix_from_float(ad[lane]));
+  }
+  const double se2 = wave_sum(epol), sv = wave_sum(evdw), sl = wave_sum(eel);
+  if (lane == 0) {
+    e_part[4 * blockIdx.x + 0] = 2.0 * se2;  // PD adds each cross term twice (gbff.cpp:1655-1658)
+    e_part[4 * blockIdx.x + 1] = sv;
+    e_part[4 * blockIdx.x + 2] = sl;
+    e_part[4 * blockIdx.x + 3] = 0.0;
+  }
+}
+
+// w_i = gpi * dE/da_i * P4 with gpi = a_i^2 / 166.0 (gbff.cpp:1772)
+__global__ void k_gb_chain_weights(int n, float P4, const double *__restrict__ born, const long long *__restrict__ deda,
+                                   float *__restrict__ wgt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = born[i];
+  const double gpi = -(a * a) / -166.0;
+  wgt[i] = (float)(gpi * ((double)deda[i] * (1.0 / kFixScale)) * (double)P4);
+}
+
+// ---- pass 3: Born-radius chain rule ------------------------------------------------
+__global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict__ work, const GBSys *__restrict__ sys,
+                                                 const double *__restrict__ x, const double *__restrict__ y,
+                                                 const double *__restrict__ z, const float *__restrict__ gbrad,
+                                                 const float *__restrict__ Vj, const float *__restrict__ wgt,
+                                                 const int8_t *__restrict__ band, const int *__restrict__ sp_off,
+                                                 const int *__restrict__ sp_part, const int *__restrict__ sp_ord,
+                                                 long long *__restrict__ fx, long long *__restrict__ fy,
+                                                 long long *__restrict__ fz) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sR[kTile], sV[kTile], sW[kTile];
+  __shared__ float ax[kTile], ay[kTile], az[kTile];
+  const GBWork w = work[blockIdx.x];
+  const GBSys S = sys[w.sys];
+  const int lane = threadIdx.x;
+  const int end = S.first + S.count;
+  const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
+  const bool diag = w.i0 == w.j0;
+  {
+    const int j = w.j0 + lane;
+    const bool v = lane < nj;
+    sx[lane] = v ? x[j] : 0.0;
+    sy[lane] = v ? y[j] : 0.0;
+    sz[lane] = v ? z[j] : 0.0;
+    sR[lane] = v ? gbrad[j] : 0.0f;
+    sV[lane] = v ? Vj[j] : 0.0f;
+    sW[lane] = v ? wgt[j] : 0.0f;
+    ax[lane] = ay[lane] = az[lane] = 0.0f;
+  }
+  const int i = w.i0 + lane;
+  const bool iv = lane < ni;
+  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
+  const float Ri = iv ? gbrad[i] : 0.0f, Vi = iv ? Vj[i] : 0.0f, Wi = iv ? wgt[i] : 0.0f;
+  __syncthreads();
+  float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
+  for (int t = 0; t < kTile; t++) {
+    const int jj = (lane + t) & (kTile - 1);
+    bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
+    double dx, dy, dz, r2;
+    {
+#pragma clang fp contract(off)
+      dx = sx[jj] - xi;
+      dy = sy[jj] - yi;
+      dz = sz[jj] - zi;
+      r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1789-1792
+    }
+    on = on && r2 <= K.still2;
+    if (w.special && on) {
+      const int bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off,
+                                   sp_off + S.sp_base, sp_part, sp_ord);
+      on = bo > 2;
+    }
+    const float r2f = on ? (float)r2 : 1.0f;
+    const float Rs = Ri + sR[jj];
+    const float rdivR = r2f / (Rs * Rs);
+    float ccf = 1.0f, dccf = 0.0f;
+    if (rdivR <= K.invP5) {
+      const float th = rdivR * K.P5 * K.pi;
+      float sn, cs;
+      sincosf(th, &sn, &cs);
+      const float sq = 0.5f - 0.5f * cs;
+      ccf = sq * sq;
+      dccf = 2.0f * sq * sn * th;
+    }
+    // both directions of the pair: gpi deda_i P4 Vj + gpj deda_j P4 Vi
+    const float r6 = r2f * r2f * r2f;
+    const float dedr = on ? (Wi * sV[jj] + sW[jj] * Vi) * (4.0f * ccf - dccf) / r6 : 0.0f;
+    const float gx = dedr * (float)dx, gy = dedr * (float)dy, gz = dedr * (float)dz;
+    fix -= gx;  // dedi_i += dedr (p_j - p_i); force = -dedi
+    fiy -= gy;
+    fiz -= gz;
+    ax[jj] += gx;
+    ay[jj] += gy;
+    az[jj] += gz;
+  }
+  __syncthreads();
+  if (iv) {
+    fix_add(&fx[i], fix_from_float(fix));
+    fix_add(&fy[i], fix_from_float(fiy));
+    fix_add(&fz[i], fix_from_float(fiz));
+  }
+  if (lane < nj) {
+    const int j = w.j0 + lane;
+    fix_add(&fx[j], fix_from_float(ax[lane]));
+    fix_add(&fy[j], fix_from_float(ay[lane]));
+    fix_add(&fz[j], fix_from_float(az[lane]));
+  }
+}
+
+__global__ void k_zero_ll1(long long *__restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// Per-system sums of per-item partials: out[4*s + d] = sum over items [beg_s, end_s).
+__global__ void k_seg_reduce4(const double *__restrict__ part, const int *__restrict__ beg,
+                              const int *__restrict__ endp, double *__restrict__ out, int out_stride,
+                              int out_off) {
+  __shared__ double s[4][256];
+  const int sysi = blockIdx.x;
+  double a[4] = {0, 0, 0, 0};
+  for (int k = beg[sysi] + threadIdx.x; k < endp[sysi]; k += 256)
+    for (int d = 0; d < 4; d++) a[d] += part[4 * k + d];
+  for (int d = 0; d < 4; d++) s[d][threadIdx.x] = a[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 4; d++) s[d][threadIdx.x] += s[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[(size_t)sysi * out_stride + out_off + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Per-system sum of a per-atom array into slot d of the system's output.
+__global__ void k_seg_sum_atoms(const double *__restrict__ v, const int *__restrict__ first,
+                                const int *__restrict__ count, double *__restrict__ out, int out_stride,
+                                int out_off) {
+  __shared__ double s[256];
+  const int sysi = blockIdx.x;
+  double a = 0.0;
+  for (int k = threadIdx.x; k < count[sysi]; k += 256) a += v[first[sysi] + k];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)sysi * out_stride + out_off] = s[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Host side.  GBState holds the device arrays of one or more systems concatenated.
+
+GBState::~GBState() = default;
+
+// FF_GeneralizedBorn_Still::calcFixedBornRadiiTerms (gbff.cpp:324-398), FP64 on the host.
+void gb_fixed_terms(int n, const double *gbrad, const pdgpu_gb_params &p, const int32_t *off12,
+                    const int32_t *idx12, const double *d12, const int32_t *off13, const int32_t *idx13,
+                    const double *d13, std::vector<double> &Vj, std::vector<double> &terms123) {
+  const double PI = Consts::PI, FourPI = 4.0 * PI;
+  auto sqr = [](double v) { return v * v; };
+  Vj.assign(n, 0.0);
+  terms123.assign(n, 0.0);
+  for (int j = 0; j < n; j++) {
+    double Rj = gbrad[j];
+    if (Rj <= 0.01) Rj = 1.15;  // Qiu et al. radius for hydrogens (gbff.cpp:343-344)
+    Vj[j] = FourPI * Rj * Rj * Rj / 3;
+    for (int nk = off12[j]; nk < off12[j + 1]; nk++) {
+      const int k = idx12[nk];
+      double Rk = gbrad[k];
+      if (Rk <= 0.01) Rk = 1.15;
+      const double rjk = 1.01 * d12[nk];
+      const double hjk = Rj * (1.0 + (sqr(Rk) - sqr(Rj) - sqr(rjk)) / (2.0 * Rj * rjk));
+      Vj[j] -= PI * sqr(hjk) * (3 * Rj - hjk) / 3;
+    }
+    if (Vj[j] < 0) Vj[j] = 0;
+  }
+  for (int i = 0; i < n; i++) {
+    double Rj = gbrad[i];
+    if (Rj <= 0.01) Rj = 1.15;
+    const double term1 = -166.0 / (Rj + p.DielectricOffset + p.P1);
+    double term2 = 0;
+    for (int nj = off12[i]; nj < off12[i + 1]; nj++)
+      term2 += p.P2 * Vj[idx12[nj]] / (sqr(d12[nj]) * sqr(d12[nj]));
+    double term3 = 0;
+    for (int nj = off13[i]; nj < off13[i + 1]; nj++)
+      term3 += p.P3 * Vj[idx13[nj]] / (sqr(d13[nj]) * sqr(d13[nj]));
+    terms123[i] = term1 + term2 + term3;
+  }
+}
+
+void GBState::set_params(const pdgpu_gb_params &p) {
+  prm = p;
+  const double ec = Consts::econv();
+  GBK k{};
+  k.cut2 = p.Cutoff * p.Cutoff;
+  k.still2 = p.GbsaStillCutoff * p.GbsaStillCutoff;
+  k.vdwcut2 = p.VdwCutoff * p.VdwCutoff;
+  k.premul = (float)(-Consts::halfeconv() * (1 / p.DielectricSolute - 1 / p.DielectricSolvent));
+  k.ec_in = (float)(ec / p.DielectricSolute);
+  k.ec_out = (float)(ec / p.DielectricSolvent);
+  const double w = p.Cutoff - p.InnerCutoff;
+  k.inner = (float)p.InnerCutoff;
+  k.invw = w > 0 ? (float)(1.0 / w) : 0.0f;
+  k.invw2_4 = w > 0 ? (float)(4.0 / (w * w)) : 0.0f;
+  const double vw = p.VdwCutoff - p.VdwInnerCutoff;
+  k.vdwinner = (float)p.VdwInnerCutoff;
+  k.vinvw = vw > 0 ? (float)(1.0 / vw) : 0.0f;
+  k.vinvw2_4 = vw > 0 ? (float)(4.0 / (vw * vw)) : 0.0f;
+  k.thresh = (float)p.ExpApproxThreshold;
+  k.P4 = (float)p.P4;
+  k.P5 = (float)p.P5;
+  k.invP5 = (float)(1 / p.P5);
+  k.vdw14 = (float)p.Vdw14Scaling;
+  k.elec14 = (float)p.Elec14Scaling;
+  k.pi = (float)Consts::PI;
+  kp = k;
+}
+
+// Adds one system (atoms [first, first+count) of the concatenated arrays) and
+// its tile pairs to the work list.
+void GBState::add_system(int first, int count, int mid, const std::vector<int8_t> &band_sys,
+                         const std::vector<int> &spi, const std::vector<int> &spj, const std::vector<int> &spo) {
+  GBSys S{};
+  S.first = first;
+  S.count = count;
+  S.mid = mid;
+  S.band_off = (long)h_band.size();
+  S.sp_base = (int)h_sp_off.size();
+  h_band.insert(h_band.end(), band_sys.begin(), band_sys.end());
+  // sparse partners keyed by the lower local index, last one of a pair wins
+  std::vector<int> cnt(count + 1, 0);
+  for (size_t k = 0; k < spi.size(); k++) cnt[std::min(spi[k], spj[k]) + 1]++;
+  for (int i = 0; i < count; i++) cnt[i + 1] += cnt[i];
+  const int part_base = (int)h_sp_part.size();
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  h_sp_part.resize(part_base + spi.size());
+  h_sp_ord.resize(part_base + spi.size());
+  std::vector<std::vector<int>> partner_tiles((count + kTile - 1) / kTile);
+  for (size_t k = 0; k < spi.size(); k++) {
+    const int lo = std::min(spi[k], spj[k]), hi = std::max(spi[k], spj[k]);
+    h_sp_part[part_base + fill[lo]] = hi;
+    h_sp_ord[part_base + fill[lo]] = spo[k];
+    fill[lo]++;
+  }
+  for (int i = 0; i <= count; i++) h_sp_off.push_back(part_base + cnt[i]);
+  const int sysi = (int)h_sys.size();
+  h_sys.push_back(S);
+  // tile pairs (upper triangle); special when bond orders can differ from 7
+  const int nt = (count + kTile - 1) / kTile;
+  std::vector<char> sp_tile((size_t)nt * nt, 0);
+  for (size_t k = 0; k < spi.size(); k++) {
+    const int a = std::min(spi[k], spj[k]) / kTile, b = std::max(spi[k], spj[k]) / kTile;
+    sp_tile[(size_t)a * nt + b] = 1;
+  }
+  h_work_beg.push_back((int)h_work.size());
+  for (int ti = 0; ti < nt; ti++)
+    for (int tj = ti; tj < nt; tj++) {
+      const int i0 = ti * kTile, j0 = tj * kTile;
+      const int gap = j0 - (i0 + kTile - 1);  // smallest j - i over the tile pair
+      GBWork wk{sysi, first + i0, first + j0, (gap < mid || sp_tile[(size_t)ti * nt + tj]) ? 1 : 0};
+      h_work.push_back(wk);
+    }
+  h_work_end.push_back((int)h_work.size());
+  n_atoms_total = std::max(n_atoms_total, first + count);
+}
+
+void GBState::upload(hipStream_t s) {
+  work.upload(h_work.data(), h_work.size(), s);
+  sysd.upload(h_sys.data(), h_sys.size(), s);
+  if (h_band.empty()) h_band.push_back(7);
+  band.upload(h_band, s);
+  sp_off.upload(h_sp_off, s);
+  if (h_sp_part.empty()) {
+    h_sp_part.push_back(0);
+    h_sp_ord.push_back(0);
+  }
+  sp_part.upload(h_sp_part, s);
+  sp_ord.upload(h_sp_ord, s);
+  work_beg.upload(h_work_beg, s);
+  work_end.upload(h_work_end, s);
+  std::vector<int> f, c;
+  for (auto &S : h_sys) {
+    f.push_back(S.first);
+    c.push_back(S.count);
+  }
+  sys_first.upload(f, s);
+  sys_count.upload(c, s);
+  const int n = n_atoms_total;
+  gpol.resize(n);
+  deda.resize(n);
+  born.resize(n);
+  bornf.resize(n);
+  wgt.resize(n);
+  eself.resize(n);
+  e_part.resize(h_work.size() * 4);
+}
+
+// Runs the three passes.  Forces are added to fx/fy/fz (fixed point, kcal/mol/A);
+// per-system energies (kcal/mol) go to out[s*stride + off + {0: pol cross, 1: vdw,
+// 2: elec, 3: -}] and out[s*stride + off + 4] (pol self).
+void GBState::run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
+                  long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner) {
+  const int n = n_atoms_total;
+  const int nw = (int)h_work.size();
+  const int nsys = (int)h_sys.size();
+  const GBK &K = kp;
+  const GBWork *W = work.p;
+  const GBSys *SY = sysd.p;
+  hipLaunchKernelGGL(k_zero_ll1, dim3((n + 255) / 256), dim3(256), 0, s, gpol.p, n);
+  {
+    ScopedTimerOpt t(timer_owner, "gb_radii");
+    hipLaunchKernelGGL(k_gb_radii, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, gbrad.p, Vj.p, band.p, sp_off.p,
+                       sp_part.p, sp_ord.p, gpol.p);
+  }
+  hipLaunchKernelGGL(k_gb_finalize_radii, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::halfeconv(),
+                     prm.BornRadiusOffset, (double)kp.premul, terms123.p, gpol.p, q.p, born.p, bornf.p, deda.p,
+                     eself.p);
+  {
+    ScopedTimerOpt t(timer_owner, "gb_pair");
+    hipLaunchKernelGGL(k_gb_pair, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, q.p, ljr.p, lje.p, bornf.p, band.p,
+                       sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz, deda.p, e_part.p);
+  }
+  hipLaunchKernelGGL(k_gb_chain_weights, dim3((n + 255) / 256), dim3(256), 0, s, n, kp.P4, born.p, deda.p, wgt.p);
+  {
+    ScopedTimerOpt t(timer_owner, "gb_chain");
+    hipLaunchKernelGGL(k_gb_chain, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, gbrad.p, Vj.p, wgt.p, band.p,
+                       sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz);
+  }
+  PDGPU_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_seg_reduce4, dim3(nsys), dim3(256), 0, s, e_part.p, work_beg.p, work_end.p, out, out_stride,
+                     out_off);
+  hipLaunchKernelGGL(k_seg_sum_atoms, dim3(nsys), dim3(256), 0, s, eself.p, sys_first.p, sys_count.p, out,
+                     out_stride, out_off + 4);
+}
+
+// ---- single-system glue ------------------------------------------------------------
+
+void Engine::launch_gb(double *energy_slots) {
+  require(have_gb, "GB component not set up");
+  gb->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, energy_slots, 8, 0, this);
+}
+
+void Engine::gb_energies_to_host(const double *h, pdgpu_energies &e) {
+  // h: [0] cross (2x pair sum), [1] vdw, [2] elec, [3] -, [4] self, kcal/mol
+  const double kj = Consts::kcal_to_J();
+  const double pol = h[0] + h[4];
+  e.epot_vdw += h[1] * kj;
+  e.epot_elec += h[2] * kj;
+  e.epot_pol += pol * kj;
+  e.epot_pol_cross += h[0] * kj;
+  e.epot_pol_self += h[4] * kj;
+  e.epot += (h[1] + h[2] + pol) * kj;
+}
 
 }  // namespace pdgpu
 
-extern "C" int pdgpu_setup_gb(pdgpu_engine *, const pdgpu_gb_params *, const double *, const int32_t *,
-                              const int32_t *, const double *, const int32_t *, const int32_t *, const double *) {
-  return PDGPU_ERR_SETUP;
+using pdgpu::Engine;
+
+extern "C" int pdgpu_setup_gb(pdgpu_engine *h, const pdgpu_gb_params *p, const double *gbradius,
+                              const int32_t *off12, const int32_t *idx12, const double *d12, const int32_t *off13,
+                              const int32_t *idx13, const double *d13) {
+  Engine *E = reinterpret_cast<Engine *>(h);
+  if (!E) return PDGPU_ERR_ARG;
+  try {
+    PDGPU_HIP(hipSetDevice(E->device));
+    E->require(p && gbradius && off12 && off13, "null GB argument");
+    E->require(!E->h_charge.empty(), "atom parameters not set");
+    E->require(p->DielectricSolute > 0 && p->DielectricSolvent > 0 && p->P5 > 0, "bad GB parameters");
+    const int n = E->n;
+    for (int i = 0; i < n; i++) {
+      for (int k = off12[i]; k < off12[i + 1]; k++) E->require(idx12[k] >= 0 && idx12[k] < n, "cov12 index");
+      for (int k = off13[i]; k < off13[i + 1]; k++) E->require(idx13[k] >= 0 && idx13[k] < n, "cov13 index");
+    }
+    std::vector<double> Vj, t123;
+    pdgpu::gb_fixed_terms(n, gbradius, *p, off12, idx12, d12, off13, idx13, d13, Vj, t123);
+    auto gb = std::make_unique<pdgpu::GBState>();
+    gb->set_params(*p);
+    gb->add_system(0, n, E->mid, E->h_band, E->h_sp_i, E->h_sp_j, E->h_sp_order);
+    gb->upload(E->stream);
+    std::vector<float> q(n), r(n), e(n), g(n), v(n);
+    const double es = 1.0 / std::sqrt(pdgpu::Consts::kcal_to_J());
+    for (int i = 0; i < n; i++) {
+      q[i] = (float)E->h_charge[i];
+      r[i] = (float)E->h_radius[i];
+      e[i] = (float)(E->h_epsilon[i] * es);
+      g[i] = (float)gbradius[i];
+      v[i] = (float)Vj[i];
+    }
+    gb->q.upload(q, E->stream);
+    gb->ljr.upload(r, E->stream);
+    gb->lje.upload(e, E->stream);
+    gb->gbrad.upload(g, E->stream);
+    gb->Vj.upload(v, E->stream);
+    gb->terms123.upload(t123, E->stream);
+    E->sync();
+    E->gb = std::move(gb);
+    E->have_gb = true;
+    // FF_NonBonded::setup requests the cutoffs; GB also wants the shadow list (gbff.cpp:97-101)
+    E->list_cutoff = std::max(E->list_cutoff, std::max(p->Cutoff, p->VdwCutoff));
+    return PDGPU_OK;
+  } catch (const pdgpu::EngineError &x) {
+    E->last_error = x.what();
+    return x.code;
+  } catch (const std::exception &x) {
+    E->last_error = x.what();
+    return PDGPU_ERR_ARG;
+  }
 }
 
-extern "C" int pdgpu_get_born_radii(pdgpu_engine *, double *) { return PDGPU_ERR_SETUP; }
+extern "C" int pdgpu_get_born_radii(pdgpu_engine *h, double *radii) {
+  Engine *E = reinterpret_cast<Engine *>(h);
+  if (!E || !radii) return PDGPU_ERR_ARG;
+  if (!E->have_gb) {
+    E->last_error = "GB component not set up";
+    return PDGPU_ERR_ARG;
+  }
+  try {
+    E->gb->born.download(radii, E->n, E->stream);
+    E->sync();
+    return PDGPU_OK;
+  } catch (const pdgpu::EngineError &x) {
+    E->last_error = x.what();
+    return x.code;
+  }
+}

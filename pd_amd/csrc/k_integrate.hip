@@ -225,9 +225,9 @@ __global__ void k_record(const double *__restrict__ e_out, int use_nb, int use_b
   double ep = 0.0;
   if (use_nb) ep += e_out[0] + e_out[1] + vdwcor;
   if (use_bonded) ep += e_out[4] + e_out[5] + e_out[6];
-  if (use_gb) ep += e_out[8] + e_out[9] + e_out[10] + e_out[11];
+  if (use_gb) ep += e_out[8] + e_out[9] + e_out[10] + e_out[12];
   trace[2 * s] = ep * kcal_to_J;
-  trace[2 * s + 1] = e_out[12];
+  trace[2 * s + 1] = e_out[16];
 }
 
 }  // namespace
@@ -238,7 +238,7 @@ void reduce4(hipStream_t s, const double *part, int count, double *out) {
 
 // e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
 // bond, angle, torsion, virial; [8..11] GB pol, vdw, elec, virial (see k_gb.hip);
-// [12] kinetic energy (J); [13..15] spare.
+// [12] GB self; [16] kinetic energy (J); others spare.
 void Engine::zero_forces() {
   fbx.resize(n);
   fby.resize(n);
@@ -247,8 +247,8 @@ void Engine::zero_forces() {
 }
 
 void Engine::evaluate(uint32_t components) {
-  e_out.resize(16);
-  PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 16 * sizeof(double), stream));
+  e_out.resize(24);
+  PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 24 * sizeof(double), stream));
   zero_forces();
   if (components & PDGPU_NONBONDED) {
     if (!list_valid) build_list();
@@ -295,8 +295,8 @@ void Engine::forces_to_host(double *f, size_t stride_bytes) {
 }
 
 void Engine::energies_to_host(uint32_t components, pdgpu_energies *out) {
-  double h[16];
-  e_out.download(h, 16, stream);
+  double h[24];
+  e_out.download(h, 24, stream);
   sync();
   const double kj = Consts::kcal_to_J();
   pdgpu_energies e{};
@@ -399,7 +399,7 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
     const int step = p.step_offset + s;
     if ((comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0) build_list();
     evaluate(comp);
-    integrate(p, step, e_out.p + 12);
+    integrate(p, step, e_out.p + 16);
     hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, stream, e_out.p, (comp & PDGPU_NONBONDED) ? 1 : 0,
                        (comp & PDGPU_BONDED) ? 1 : 0, (comp & PDGPU_GB) ? 1 : 0, cor, Consts::kcal_to_J(),
                        trace.p, s);
