@@ -1,0 +1,506 @@
+"""Pins the FP64 oracle (CPU only).
+
+PD ships no tests, fixtures or golden vectors for this path, so every oracle
+function is checked here against something independent of its own code:
+closed forms, an independent brute-force implementation, invariants, finite
+differences, known-answer vectors, or a second oracle path.  Each check is
+chosen so that a dropped term, a wrong sign, index or unit fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pd_amd import systems as S
+from pd_amd import units as U
+
+
+# ---------------------------------------------------------------- constants / RNG
+
+def test_constants_match_independent_values():
+    c = O.constants()
+    # econv = Na e^2 / (4 pi eps0 1A) / 4184 with PD's constants: 332.0637... kcal A / mol
+    e, e0, na = 1.60217645e-19, 8.854187817e-12, 6.0221418e23
+    econv = na * e * e / (4 * math.pi * e0 * 1e-10) / 4184.0
+    assert c["econv"] == pytest.approx(econv, rel=1e-14)
+    assert c["econv"] == pytest.approx(332.0637, rel=2e-6)
+    assert c["halfeconv_joule"] * 2 == pytest.approx(c["econv_joule"], rel=1e-15)
+    assert c["econv_joule"] * c["J2kcal"] * c["Na"] == pytest.approx(c["econv"], rel=1e-14)
+    assert U.ECONV == pytest.approx(c["econv"], rel=1e-15)
+
+
+def test_philox_known_answers():
+    # Random123 philox4x32_10 known-answer vectors
+    assert O.philox4x32([0, 0, 0, 0], [0, 0]).tolist() == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox4x32([0xffffffff] * 4, [0xffffffff] * 2).tolist() == [0x408f276d, 0x41c83b0e, 0xa20bc7c6,
+                                                                          0x6d5451fd]
+    assert O.philox4x32([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]).tolist() == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_box_muller_and_noise_statistics():
+    a, b = O.box_muller(0.25, 0.125)
+    l = math.sqrt(-2 * math.log(0.25))
+    assert a == pytest.approx(l * math.cos(-2 * math.pi * 0.125), rel=1e-15)
+    assert b == pytest.approx(l * math.sin(-2 * math.pi * 0.125), rel=1e-15)
+    z = np.array([O.langevin_noise(7, 0, s, a_) for s in range(40) for a_ in range(250)]).ravel()
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.03
+    assert abs(np.corrcoef(z[:-1], z[1:])[0, 1]) < 0.03
+    assert not np.array_equal(O.langevin_noise(7, 0, 1, 3), O.langevin_noise(7, 0, 2, 3))
+
+
+# ---------------------------------------------------------------- space / cleanSpace
+
+def test_basis_vectors_layout():
+    box = np.array([20.0, 21.0, 22.0])
+    b = O.basis_vectors(True, box, 9.0)
+    for i in (-1, 0, 1):
+        for j in (-1, 0, 1):
+            for k in (-1, 0, 1):
+                assert b[(i + 1) * 9 + (j + 1) * 3 + (k + 1)].tolist() == [i * 20.0, j * 21.0, k * 22.0]
+    with pytest.raises(ValueError):
+        O.basis_vectors(True, box, 10.5)  # beyond half of the smallest edge
+    assert O.basis_vectors(False, None, 100.0)[0].tolist() == [0.0, 0.0, 0.0]
+
+
+def test_clean_space_moves_molecule_centres():
+    s = S.water_box(64, seed=1)
+    L = s.box[0]
+    pos = s.pos.copy()
+    pos[0:3] += [L, 0.0, 0.0]        # molecule 0 one box to the right
+    pos[3:6] += [0.0, -2 * L, L]     # molecule 1 two boxes down, one forward
+    out = O.clean_space(s, pos)
+    np.testing.assert_array_equal(out[6:], pos[6:])   # molecules with COG inside are untouched
+    np.testing.assert_allclose(out[0:3], s.pos[0:3], atol=1e-12)
+    np.testing.assert_allclose(out[3:6], s.pos[3:6], atol=1e-12)
+    cog = out.reshape(-1, 3, 3).mean(axis=1)
+    assert np.all(np.abs(cog) <= L / 2 + 1e-9)
+
+
+# ---------------------------------------------------------------- bond orders
+
+class _Graph:
+    def __init__(self, n, bonds, resid=None):
+        self.n = n
+        self.bonds = np.array(bonds, dtype=np.int32)
+        self.bond_l = np.ones(len(bonds))
+        nb = [[] for _ in range(n)]
+        for a, b in bonds:
+            nb[a].append(b); nb[b].append(a)
+        ang = [(i, a, j) for a in range(n) for x, i in enumerate(nb[a]) for j in nb[a][x + 1:]]
+        self.angles = np.array(ang, dtype=np.int32).reshape(-1, 3)
+        self.resid = np.zeros(n, np.int32) if resid is None else np.asarray(resid, np.int32)
+        self.cov = S.build_covalent(n, self.bonds, self.bond_l, self.angles, np.full(len(ang), 1.9))
+
+
+def _bo(g, i, j, bo):
+    a, b = min(i, j), max(i, j)
+    if b - a < bo.mid:
+        v = int(bo.band[a * bo.mid + (b - a - 1)])
+    else:
+        v = 7
+    for k in range(len(bo.sp_i)):
+        if {int(bo.sp_i[k]), int(bo.sp_j[k])} == {a, b}:
+            v = int(bo.sp_order[k])
+    return v
+
+
+def test_bond_order_chain():
+    g = _Graph(6, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)])
+    bo = O.bond_order(g)
+    assert bo.mid == 4  # largest 1-4 index delta (3) + 1
+    for i in range(6):
+        for j in range(i + 1, 6):
+            expect = {1: 1, 2: 2, 3: 3}.get(j - i, 7)
+            assert _bo(g, i, j, bo) == expect, (i, j)
+
+
+def test_bond_order_precedence_and_sparse():
+    # triangle: (0,2) is both 1-2 and 1-3 -> 1-2 wins (written last, bondorder.cpp:208-210)
+    g = _Graph(3, [(0, 1), (1, 2), (0, 2)])
+    assert _bo(g, 0, 2, O.bond_order(g)) == 1
+    # a bond between residues far apart in index is kept off the band
+    n = 14
+    bonds = [(k, k + 1) for k in range(n - 1)] + [(0, 12)]
+    resid = [k // 2 for k in range(n)]
+    g = _Graph(n, bonds, resid)
+    bo = O.bond_order(g)
+    assert bo.mid == 4
+    assert (0, 12, 1) in zip(bo.sp_i.tolist(), bo.sp_j.tolist(), bo.sp_order.tolist())
+
+
+# ---------------------------------------------------------------- neighbour list
+
+def brute_force_list(s, pos, cutoff, padding, shadow, bo):
+    """Independent vectorised restatement of calcNewList's pair test (single wrap)."""
+    n = s.n
+    lim = (cutoff + padding) ** 2
+    periodic = s.box is not None
+    counts, out = [], []
+    for i in range(n):
+        js = np.arange(n if shadow else i)
+        d = pos[js] - pos[i]
+        img = np.full(len(js), 13)
+        if periodic:
+            for ax, w in ((0, 9), (1, 3), (2, 1)):
+                L, h = s.box[ax], s.box[ax] / 2
+                up = d[:, ax] > h
+                d[up, ax] -= L; img[up] -= w
+                dn = d[:, ax] < -h
+                d[dn, ax] += L; img[dn] += w
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        keep = (r2 <= lim) & (js != i)
+        ent = []
+        for j, im in zip(js[keep], img[keep]):
+            o = _bo(s, i, int(j), bo)
+            w = int(j) | ((o & 7) << 24)
+            if periodic:
+                w |= (int(im) & 31) << 27
+            ent.append(w)
+        counts.append(len(ent))
+        out.extend(ent)
+    return np.array(counts, np.int32), np.array(out, np.uint32)
+
+
+@pytest.mark.parametrize("shadow", [False, True])
+def test_list_matches_brute_force_periodic(shadow):
+    s = S.water_box(343, seed=3)
+    pos = O.clean_space(s)
+    bo = O.bond_order(s)
+    c, p = O.build_list(s, pos, 9.0, 1.5, shadow, bo)
+    c2, p2 = brute_force_list(s, pos, 9.0, 1.5, shadow, bo)
+    np.testing.assert_array_equal(c, c2)
+    np.testing.assert_array_equal(p, p2)
+    j, order, img = O.decode(p)
+    assert set(np.unique(order)) <= {1, 2, 7} and img.max() <= 26
+
+
+def test_list_matches_brute_force_vacuum_peptide():
+    s = S.peptide_chain(12, seed=2)
+    bo = O.bond_order(s)
+    c, p = O.build_list(s, s.pos, 8.0, 0.5, False, bo)
+    c2, p2 = brute_force_list(s, s.pos, 8.0, 0.5, False, bo)
+    np.testing.assert_array_equal(c, c2)
+    np.testing.assert_array_equal(p, p2)
+    assert set(np.unique(O.decode(p)[1])) == {1, 2, 3, 7}
+
+
+# ---------------------------------------------------------------- nonbonded kernel
+
+def _pair_system(r, q=(0.0, 0.0), R=(1.7, 1.9), eps_kcal=(0.1, 0.2)):
+    s = S.water_box(1, seed=0)
+    s.box = None
+    s.pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0], [60.0, 60.0, 60.0]])
+    s.charge = np.array([q[0], q[1], 0.0])
+    s.radius = np.array([R[0], R[1], 0.0])
+    s.epsilon = np.sqrt(np.array([eps_kcal[0], eps_kcal[1], 0.0]) * U.KCAL2J / U.NA)
+    s.resid = np.array([0, 1, 2], np.int32)
+    s.cov = S.build_covalent(3, np.zeros((0, 2), np.int32), np.zeros(0), np.zeros((0, 3), np.int32), np.zeros(0))
+    return s
+
+
+def _nb(s, **kw):
+    p = O.nb_params(**kw)
+    bo = O.bond_order(s)
+    cut = max(p.Cutoff, p.VdwCutoff)
+    c, pk = O.build_list(s, s.pos, cut, 0.5, False, bo)
+    return O.nb_forces(s, s.pos, c, pk, p, O.basis_vectors(False, None, cut))
+
+
+def test_lj_minimum_is_minus_epsilon():
+    R = (1.7, 1.9)
+    s = _pair_system(R[0] + R[1], R=R, eps_kcal=(0.1, 0.2))
+    f, e = _nb(s, Cutoff=12.0, InnerCutoff=10.0, VdwCutoff=12.0, VdwInnerCutoff=10.0, DoElec=0)
+    assert e["epot_vdw"] * U.J_TO_KCALMOL == pytest.approx(-math.sqrt(0.1 * 0.2), rel=1e-12)
+    assert np.abs(f).max() < 1e-20  # dE/dr = 0 at the minimum
+
+
+def test_plain_coulomb_closed_form():
+    s = _pair_system(5.0, q=(0.5, -0.75))
+    f, e = _nb(s, Cutoff=12.0, InnerCutoff=10.0, VdwCutoff=12.0, VdwInnerCutoff=10.0, DoVdw=0,
+               ForceSwitch=0, EnergySwitch=0, Dielectric=2.0)
+    expect = U.ECONV_JOULE * 0.5 * -0.75 / (2.0 * 5.0)
+    assert e["epot_elec"] == pytest.approx(expect, rel=1e-13)
+    # force on atom 0 along +x (towards 1) for an attractive pair: F = -dE/dx0 = E/r per A
+    assert f[0, 0] == pytest.approx(-expect / 5.0 * 1e10, rel=1e-12)
+    assert f[0, 0] > 0 and f[1, 0] == pytest.approx(-f[0, 0], rel=1e-15)
+
+
+def _energy_curve(rs, **kw):
+    out = []
+    for r in rs:
+        s = _pair_system(r, q=(1.0, -1.0), eps_kcal=(0.15, 0.15))
+        f, e = _nb(s, **kw)
+        out.append((e["epot_vdw"] + e["epot_elec"], f[1, 0]))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("mode", ["force", "energy"])
+def test_switching_continuity_and_force_integral(mode):
+    kw = dict(Cutoff=10.0, InnerCutoff=8.0, VdwCutoff=10.0, VdwInnerCutoff=8.0,
+              ForceSwitch=1 if mode == "force" else 0, EnergySwitch=1 if mode == "energy" else 0)
+    # energy vanishes at the cutoff and is continuous at the inner cutoff
+    e_c = _energy_curve([10.0 - 1e-7], **kw)[0, 0]
+    assert abs(e_c) < 1e-12 * abs(_energy_curve([5.0], **kw)[0, 0])
+    e_in = _energy_curve([8.0 - 1e-9, 8.0 + 1e-9], **kw)
+    assert e_in[0, 0] == pytest.approx(e_in[1, 0], rel=1e-7)
+    # E(r) = -integral_r^rc F(r') dr' (force on atom 1 along +x is -dE/dr)
+    rs = np.linspace(4.0, 10.0, 6001)
+    cur = _energy_curve(rs, **kw)
+    F = cur[:, 1] * 1e-10  # J/A
+    integ = np.concatenate([[0.0], np.cumsum(0.5 * (F[1:] + F[:-1]) * np.diff(rs))])
+    # F = -dE/dr, so E(r) = E(rc) + integral_r^rc F dr' with E(rc) = 0
+    np.testing.assert_allclose(cur[:, 0], integ[-1] - integ, rtol=2e-5, atol=1e-8 * np.abs(cur[:, 0]).max())
+
+
+def test_14_scaling_and_exclusions():
+    g = _Graph(6, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5)])
+    g.pos = np.array([[0, 0, 0], [1.5, 0, 0], [2.3, 1.2, 0], [3.8, 1.3, 0.2], [4.5, 2.6, 0.1], [6.0, 2.8, 0.5]])
+    g.radius = np.full(6, 1.8)
+    g.epsilon = np.full(6, math.sqrt(0.1 * U.KCAL2J / U.NA))
+    g.charge = np.array([0.3, -0.3, 0.2, -0.2, 0.4, -0.4])
+    g.box = None
+    base = dict(Cutoff=12.0, InnerCutoff=10.0, VdwCutoff=12.0, VdwInnerCutoff=10.0, ForceSwitch=0, EnergySwitch=0)
+    _, e1 = _nb(g, **base, Vdw14Scaling=1.0, Elec14Scaling=1.0)
+    _, e2 = _nb(g, **base, Vdw14Scaling=0.5, Elec14Scaling=0.25)
+    # only the 1-4 pairs (0,3), (1,4), (2,5) change; 1-2/1-3 pairs never contribute
+    def pair_e(i, j):
+        r = np.linalg.norm(g.pos[i] - g.pos[j])
+        eps = g.epsilon[i] * g.epsilon[j]
+        B = ((g.radius[i] + g.radius[j]) / r) ** 6
+        return eps * (B * B - 2 * B), U.ECONV_JOULE * g.charge[i] * g.charge[j] / r
+    v14 = sum(pair_e(i, i + 3)[0] for i in range(3))
+    c14 = sum(pair_e(i, i + 3)[1] for i in range(3))
+    assert e1["epot_vdw"] - e2["epot_vdw"] == pytest.approx(0.5 * v14, rel=1e-10)
+    assert e1["epot_elec"] - e2["epot_elec"] == pytest.approx(0.75 * c14, rel=1e-10)
+    far = [(i, j) for i in range(6) for j in range(i + 4, 6)]
+    full = sum(pair_e(i, j)[0] for i, j in far) + v14
+    assert e1["epot_vdw"] == pytest.approx(full, rel=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["force", "energy", "plain"])
+def test_nonbonded_forces_are_gradients_pbc(mode):
+    s = S.water_box(216, seed=4, box_len=19.5)
+    pos = O.clean_space(s)
+    kw = dict(Cutoff=9.0, InnerCutoff=7.0, VdwCutoff=9.0, VdwInnerCutoff=7.0,
+              ForceSwitch=int(mode == "force"), EnergySwitch=int(mode == "energy"))
+    p = O.nb_params(**kw)
+    bo = O.bond_order(s)
+    c, pk = O.build_list(s, pos, 9.0, 1.0, False, bo)
+    basis = O.basis_vectors(True, s.box, 9.0)
+    f, e = O.nb_forces(s, pos, c, pk, p, basis)
+    assert np.abs(f.sum(axis=0)).max() < 1e-12 * np.abs(f).max()
+    rng = np.random.default_rng(0)
+    for a in rng.choice(s.n, 4, replace=False):
+        for d in range(3):
+            h = 1e-6
+            pp = pos.copy(); pp[a, d] += h
+            pm = pos.copy(); pm[a, d] -= h
+            ep = O.nb_forces(s, pp, c, pk, p, basis)[1]["epot"]
+            em = O.nb_forces(s, pm, c, pk, p, basis)[1]["epot"]
+            fd = -(ep - em) / (2 * h * 1e-10)
+            assert fd == pytest.approx(f[a, d], rel=2e-5, abs=1e-6 * np.abs(f).max())
+
+
+def test_virial_is_scaling_derivative():
+    s = S.peptide_chain(8, seed=1)
+    kw = dict(Cutoff=30.0, InnerCutoff=25.0, VdwCutoff=30.0, VdwInnerCutoff=25.0, ForceSwitch=0, EnergySwitch=1)
+    p = O.nb_params(**kw)
+    bo = O.bond_order(s)
+    c, pk = O.build_list(s, s.pos, 30.0, 0.5, False, bo)
+    basis = O.basis_vectors(False, None, 30.0)
+    _, e = O.nb_forces(s, s.pos, c, pk, p, basis)
+    h = 1e-7
+    ep = O.nb_forces(s, s.pos * (1 + h), c, pk, p, basis)[1]["epot"]
+    em = O.nb_forces(s, s.pos * (1 - h), c, pk, p, basis)[1]["epot"]
+    assert e["virial"] == pytest.approx((ep - em) / (2 * h), rel=1e-5)
+
+
+# ---------------------------------------------------------------- GB Still
+
+def _ions(r, q=(1.0, -1.0), gbr=(1.5, 1.7)):
+    s = _pair_system(r, q=q)
+    s.pos = s.pos[:2].copy()
+    for k in ("charge", "radius", "epsilon", "mass", "resid"):
+        setattr(s, k, getattr(s, k)[:2].copy())
+    s.gbrad = np.array(gbr, dtype=np.float64)
+    s.cov = S.build_covalent(2, np.zeros((0, 2), np.int32), np.zeros(0), np.zeros((0, 3), np.int32), np.zeros(0))
+    return s
+
+
+def _gb(s, **kw):
+    p = O.gb_params(**kw)
+    bo = O.bond_order(s)
+    cut = max(p.Cutoff, p.VdwCutoff)
+    c, pk = O.build_list(s, s.pos, cut, 0.5, True, bo)
+    Vj, t123, br0 = O.gb_fixed_terms(s, p)
+    born = O.gb_radii(s, s.pos, Vj, t123, p, c, pk)
+    f, e = O.gb_forces_fast(s, s.pos, Vj, born, p, c, pk)
+    return p, Vj, t123, born, f, e, (c, pk)
+
+
+def test_gb_isolated_atom_closed_forms():
+    s = _ions(500.0, q=(0.6, 0.0))
+    p, Vj, t123, born, f, e, _ = _gb(s, GbsaStillCutoff=10.0, Cutoff=12.0, InnerCutoff=10.0,
+                                     VdwCutoff=12.0, VdwInnerCutoff=10.0)
+    R = 1.5
+    assert Vj[0] == pytest.approx(4.0 / 3.0 * math.pi * R ** 3, rel=1e-15)
+    a = U.HALFECONV * (R + p.DielectricOffset + p.P1) / 166.0
+    assert born[0] == pytest.approx(a, rel=1e-14)
+    premul = -U.HALFECONV_JOULE * (1.0 - 1.0 / 80.0)
+    assert e["epot_pol_self"] == pytest.approx(premul * 0.36 / born[0], rel=1e-13)
+    assert e["epot_pol_cross"] == 0.0
+
+
+def test_gb_pair_closed_form_and_far_branch():
+    # near pair: Still cross term, added twice (gbff.cpp:1655-1658)
+    s = _ions(4.0)
+    p, Vj, t123, born, f, e, _ = _gb(s, Cutoff=50.0, InnerCutoff=40.0, VdwCutoff=50.0, VdwInnerCutoff=40.0,
+                                     GbsaStillCutoff=50.0, DielectricSolvent=78.5)
+    premul = -U.HALFECONV_JOULE * (1.0 - 1.0 / 78.5)
+    aa = born[0] * born[1]
+    cross = premul * -1.0 / math.sqrt(16.0 + aa * math.exp(-16.0 / (4 * aa)))
+    assert e["epot_pol_cross"] == pytest.approx(2 * cross, rel=1e-13)
+    assert e["epot_elec"] == pytest.approx(U.ECONV_JOULE * -1.0 / 4.0, rel=1e-13)
+    # far pair (uGB <= -4): no cross term and Coulomb with eps_solvent (gbff.cpp:1659-1678)
+    s = _ions(30.0)
+    p, Vj, t123, born, f, e, _ = _gb(s, Cutoff=50.0, InnerCutoff=40.0, VdwCutoff=50.0, VdwInnerCutoff=40.0,
+                                     GbsaStillCutoff=50.0, DielectricSolvent=78.5)
+    assert -30.0 ** 2 / (4 * born[0] * born[1]) <= -4.0
+    assert e["epot_pol_cross"] == 0.0
+    assert e["epot_elec"] == pytest.approx(U.ECONV_JOULE * -1.0 / (78.5 * 30.0), rel=1e-13)
+
+
+def test_gb_fastmode_matches_energy_only_path_when_compact():
+    """calcForcesIncludingVacuo and calcEnergies are different PD code paths; with
+    no pair beyond the exp threshold they must give the same energies."""
+    s = S.peptide_chain(1, seed=3, sequence=["SER"])
+    kw = dict(Cutoff=99.0, InnerCutoff=99.0, VdwCutoff=99.0, VdwInnerCutoff=99.0, GbsaStillCutoff=99.0,
+              Vdw14Scaling=0.5, Elec14Scaling=0.83333)
+    p, Vj, t123, born, f, e, (c, pk) = _gb(s, **kw)
+    d = np.linalg.norm(s.pos[:, None] - s.pos[None], axis=2)
+    assert (-(d ** 2) / (4 * np.outer(born, born))).min() > -4.0
+    e2 = O.gb_energies(s, s.pos, born, p, c, pk)
+    for k in ("epot", "epot_pol", "epot_vdw", "epot_elec"):
+        assert e2[k] == pytest.approx(e[k], rel=1e-11), k
+
+
+def test_gb_forces_are_gradients_up_to_pd_constant():
+    """FastMode forces use 166.0 where the exact radius derivative has halfeconv
+    (gbff.cpp:1772 vs 480); the force is the gradient to ~2e-4 of the chain term."""
+    s = S.peptide_chain(3, seed=6)
+    kw = dict(Cutoff=99.0, InnerCutoff=99.0, VdwCutoff=99.0, VdwInnerCutoff=99.0, GbsaStillCutoff=99.0,
+              Vdw14Scaling=0.5, Elec14Scaling=0.83333)
+    p, Vj, t123, born, f, e, (c, pk) = _gb(s, **kw)
+
+    def energy(pos):
+        br = O.gb_radii(s, pos, Vj, t123, p, c, pk)
+        return O.gb_forces_fast(s, pos, Vj, br, p, c, pk)[1]["epot"]
+    rng = np.random.default_rng(2)
+    for a in rng.choice(s.n, 5, replace=False):
+        for d in range(3):
+            h = 1e-6
+            pp = s.pos.copy(); pp[a, d] += h
+            pm = s.pos.copy(); pm[a, d] -= h
+            fd = -(energy(pp) - energy(pm)) / (2 * h * 1e-10)
+            assert fd == pytest.approx(f[a, d], rel=2e-3, abs=2e-3 * np.abs(f).max())
+
+
+# ---------------------------------------------------------------- bonded
+
+def test_bonded_closed_forms_and_gradients():
+    s = S.peptide_chain(3, seed=1)
+    rng = np.random.default_rng(3)
+    pos = s.pos + rng.normal(scale=0.05, size=s.pos.shape)
+    f, e = O.bonded_forces(s, pos)
+    d = np.linalg.norm(pos[s.bonds[:, 0]] - pos[s.bonds[:, 1]], axis=1)
+    assert e["epot_bond"] == pytest.approx(np.sum(s.bond_k * (d - s.bond_l) ** 2), rel=1e-12)
+    u = pos[s.angles[:, 0]] - pos[s.angles[:, 1]]
+    v = pos[s.angles[:, 2]] - pos[s.angles[:, 1]]
+    th = np.arccos(np.sum(u * v, 1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1))
+    assert e["epot_angle"] == pytest.approx(np.sum(s.angle_k * (th - s.angle_t0) ** 2), rel=1e-9)
+    assert np.abs(f.sum(axis=0)).max() < 1e-12 * np.abs(f).max()
+    for a in rng.choice(s.n, 5, replace=False):
+        for dd in range(3):
+            h = 1e-6
+            pp = pos.copy(); pp[a, dd] += h
+            pm = pos.copy(); pm[a, dd] -= h
+            fd = -(O.bonded_forces(s, pp)[1]["epot"] - O.bonded_forces(s, pm)[1]["epot"]) / (2 * h * 1e-10)
+            assert fd == pytest.approx(f[a, dd], rel=1e-5, abs=1e-7 * np.abs(f).max())
+
+
+def test_torsion_energy_trans_and_cis():
+    g = _Graph(4, [(0, 1), (1, 2), (2, 3)])
+    for phi_deg, n_fold in ((180.0, 3), (0.0, 1), (60.0, 3)):
+        phi = math.radians(phi_deg)
+        pos = np.array([[1.0, 1.0, 0.0], [0.0, 0.0, 0.0], [1.5, 0.0, 0.0],
+                        [2.5, math.cos(phi), math.sin(phi)]])
+        g.pos = pos
+        g.torsions = np.array([[0, 1, 2, 3]], np.int32)
+        g.tor_terms = np.array([1], np.int32)
+        V = U.kcal_to_j(1.0)
+        g.tor_vn = np.array([[V, 0, 0, 0]]); g.tor_n = np.array([[n_fold, 0, 0, 0]], float)
+        g.tor_gamma = np.zeros((1, 4))
+        g.bonds = np.zeros((0, 2), np.int32); g.angles = np.zeros((0, 3), np.int32)
+        _, e = O.bonded_forces(g, pos)
+        assert e["epot_torsion"] == pytest.approx(V * (1 + math.cos(n_fold * phi)), abs=1e-12 * V)
+
+
+# ---------------------------------------------------------------- integrators / REMD
+
+def test_velocity_verlet_pd_recurrence():
+    """PD's velocity Verlet applies three half kicks with the step's force
+    (md.cpp:820-853): v' = v + 1.5 F dt/m, x' = x + dt v' / A."""
+    rng = np.random.default_rng(1)
+    m = rng.uniform(1, 20, 5) * U.AMU
+    f = rng.normal(size=(5, 3)) * 1e-10
+    x = rng.normal(size=(5, 3))
+    v = rng.normal(size=(5, 3)) * 300.0
+    x1, v1 = x.copy(), v.copy()
+    ke = O.vv_step(m, 1e-15, f, x1, v1)
+    vh = v + 0.5 * f * 1e-15 / m[:, None]
+    assert ke == pytest.approx(np.sum(0.5 * m[:, None] * vh ** 2), rel=1e-13)
+    np.testing.assert_allclose(v1, v + 1.5 * f * 1e-15 / m[:, None], rtol=1e-14)
+    np.testing.assert_allclose(x1, x + 1e-15 * v1 / 1e-10, rtol=1e-13)
+
+
+def test_langevin_coefficients_branches_agree():
+    """PD's Taylor series (gamma*dt <= 0.05) against its own closed forms
+    (md.cpp:947-986).  c0, c1, c2, varv and varr agree; crv does not: PD's series
+    has -3x/16 at first order where the closed form expands to -x/16
+    (crv = sqrt(3)/2 (1 - x/8 + ...)), so the two differ by sqrt(3) x / 8."""
+    for gt in (0.002, 0.02, 0.05, 0.049999):
+        lo = O.langevin_coeffs(gt)
+        c0 = math.exp(-gt)
+        c1 = (1 - c0) / gt
+        c2 = (1 - c1) / gt
+        eg = math.exp(-gt)
+        varv = 1 - eg * eg
+        varr = 2 * gt - 3 + (4 - eg) * eg
+        crv = (1 - eg) ** 2 / math.sqrt(varv * varr)
+        np.testing.assert_allclose(lo[:2], [c0, c1], rtol=1e-12)
+        assert lo[2] == pytest.approx(c2, rel=1e-8)    # (1 - c1)/gt cancels in the closed form
+        assert lo[4] == pytest.approx(varv, rel=1e-10)
+        assert lo[3] == pytest.approx(varr, rel=2e-6)   # varr cancels heavily in the closed form
+        assert crv - lo[5] == pytest.approx(math.sqrt(3.0) * gt / 8.0, rel=0.1)
+    assert O.langevin_coeffs(1e-9)[5] == pytest.approx(math.sqrt(3.0) / 2.0, rel=1e-8)
+    hi = O.langevin_coeffs(0.2)
+    assert hi[0] == pytest.approx(math.exp(-0.2), rel=1e-15)
+
+
+def test_remd_metropolis():
+    kB = 1.38065e-23
+    assert O.remd_accept(-1e-19, -2e-19, 300.0, 330.0, 0.999)  # delta <= 0 always accepted
+    e_lo, e_hi, t_lo, t_hi = -2e-19, -1e-19, 300.0, 330.0
+    delta = (e_lo - e_hi) * (1 / (kB * t_hi) - 1 / (kB * t_lo))
+    assert delta > 0
+    p = math.exp(-delta)
+    assert O.remd_accept(e_lo, e_hi, t_lo, t_hi, p * 0.999)
+    assert not O.remd_accept(e_lo, e_hi, t_lo, t_hi, p * 1.001)
+
+
+def test_kinetic_energy():
+    m = np.array([2.0, 3.0])
+    v = np.array([[1.0, 2.0, 2.0], [0.0, 3.0, 4.0]])
+    assert O.kinetic_energy(m, v) == pytest.approx(0.5 * 2 * 9 + 0.5 * 3 * 25, rel=1e-15)
