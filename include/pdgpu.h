@@ -98,6 +98,7 @@ typedef struct pdgpu_md_params {
   int32_t step_offset;    /* first step number (Langevin's first/last step guards) */
   int32_t total_steps;    /* MolecularDynamics::Steps */
   uint64_t seed;          /* Langevin noise: Philox4x32-10 key */
+  uint32_t noise_stream;  /* Philox counter word 2; a batch adds the system index */
 } pdgpu_md_params;
 
 /* ---- lifecycle -------------------------------------------------------------- */
@@ -208,6 +209,43 @@ int pdgpu_get_born_radii(pdgpu_engine *e, double *radii);
  * integrator computes.  Returns PDGPU_ERR_UNSTABLE if an energy became NaN. */
 int pdgpu_md(pdgpu_engine *e, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
              double *ekin_trace);
+
+/* ---- batches of independent systems / REMD replicas ------------------------------ */
+
+/* Many independent systems of uneven size advanced together on one GPU.  PD has
+ * no batch object: its REX_Local (remd.h:127) runs replicas one after another on
+ * one WorkSpace and its protocols run one system at a time; a batch holds N such
+ * systems at once.  It is built from single-system engines that carry the full
+ * setup (atom parameters, bond orders, bonded terms, GB, positions, velocities);
+ * their data are copied, the engines stay usable.  GB parameters must be equal.
+ * Systems keep their order; per-system arrays below are indexed by it. */
+typedef struct pdgpu_batch pdgpu_batch;
+
+int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_batch **out);
+int pdgpu_batch_destroy(pdgpu_batch *b);
+const char *pdgpu_batch_last_error(const pdgpu_batch *b);
+/* Target temperature of each system (K); default 300. */
+int pdgpu_batch_set_temperatures(pdgpu_batch *b, const double *temperatures);
+int pdgpu_batch_get_positions(pdgpu_batch *b, int32_t system, double *xyz, size_t stride_bytes);
+int pdgpu_batch_set_positions(pdgpu_batch *b, int32_t system, const double *xyz, size_t stride_bytes);
+int pdgpu_batch_get_velocities(pdgpu_batch *b, int32_t system, double *xyz, size_t stride_bytes);
+int pdgpu_batch_set_velocities(pdgpu_batch *b, int32_t system, const double *xyz, size_t stride_bytes);
+/* Forces of all systems (concatenated atoms, AoS, J/m, ADDED to f_accum, may be
+ * NULL) and per-system energies (energies[n_systems], may be NULL). */
+int pdgpu_batch_compute(pdgpu_batch *b, uint32_t components, double *f_accum, pdgpu_energies *energies);
+/* n_steps MD steps of every system as one hipGraph-replayed device step.  Traces
+ * (may be NULL) receive n_steps * n_systems values, step-major (J). */
+int pdgpu_batch_md(pdgpu_batch *b, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
+                   double *ekin_trace);
+/* REX_Local::runcore (remd.cpp:65-254): n_rounds of {every replica runs
+ * p->total_steps steps; alternating even/odd neighbour exchanges with
+ * exchangeCriterion (remd.cpp:349-357)}.  System s is the replica at the s-th
+ * temperature; accepted structures swap between neighbours with velocities
+ * rescaled by sqrt(T ratio).  accepted/attempted count exchanges per pair
+ * (s, s+1); final_epot gets the last round's replica energies (J).  The
+ * Metropolis uniforms come from Philox4x32-10 keyed by `seed`. */
+int pdgpu_batch_remd(pdgpu_batch *b, const pdgpu_md_params *p, int32_t n_rounds, uint64_t seed,
+                     int32_t *accepted, int32_t *attempted, double *final_epot);
 
 /* ---- instrumentation ---------------------------------------------------------------- */
 

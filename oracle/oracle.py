@@ -129,6 +129,8 @@ def lib():
         L.pdo_philox4x32.argtypes = [_up, _up, _up]
         L.pdo_box_muller.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pdo_langevin_noise.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _dp]
+        L.pdo_remd_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+        L.pdo_remd_uniform.restype = C.c_double
         _lib = L
     return _lib
 
@@ -335,6 +337,10 @@ def langevin_noise(seed, stream, step, atom) -> np.ndarray:
     out = np.zeros(6)
     lib().pdo_langevin_noise(seed, stream, step, atom, out)
     return out
+
+
+def remd_uniform(seed, pair, rnd) -> float:
+    return lib().pdo_remd_uniform(seed, pair, rnd)
 
 
 def decode(packed):

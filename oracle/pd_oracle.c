@@ -1155,3 +1155,11 @@ void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
     pdo_box_muller(u53(w[0], w[1]), u53(w[2], w[3]), &out[2 * k], &out[2 * k + 1]);
   }
 }
+
+double pdo_remd_uniform(uint64_t seed, uint32_t pair, uint32_t round) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t ctr[4] = {pair, round, 0x52454D44u, 0u};
+  uint32_t w[4];
+  pdo_philox4x32(ctr, key, w);
+  return u53(w[0], w[1]);
+}

@@ -191,6 +191,11 @@ void pdo_box_muller(double u1, double u2, double *n1, double *n2);
  * in the order pdo_langevin_step consumes them. */
 void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t atom, double out[6]);
 
+/* The uniform deviate the device REMD uses in place of frand() for the exchange
+ * test of neighbour pair (r, r+1) in round `round`: Philox4x32-10 on counter
+ * (r, round, 0x52454D44, 0) with key `seed`, 53 bits mapped to (0,1]. */
+double pdo_remd_uniform(uint64_t seed, uint32_t pair, uint32_t round);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,6 +146,20 @@ struct GBSys {
 
 struct Engine;
 
+// Device pointers of FF_Bonded terms (k_bonded.hip); force constants in kcal/mol units.
+struct BondedArrays {
+  int nb, na, nt;
+  const int *bi, *bj;
+  const double *bl, *bk;
+  const int *ai, *aa, *aj;
+  const double *at0, *ak;
+  const int *ti, *ta, *tb, *tj, *tterms;
+  const double *tvn, *tnn, *tgamma;
+};
+void launch_bonded_kernel(hipStream_t s, const BondedArrays &B, const double *x, const double *y,
+                          const double *z, long long *fx, long long *fy, long long *fz, double *partial,
+                          double *eterm);
+
 // Device arrays of the GB passes for one or more systems laid end to end.
 struct GBState {
   pdgpu_gb_params prm{};

@@ -1,0 +1,723 @@
+// k_batch.hip -- many independent GB/SA systems (or REMD replicas) advanced together.
+//
+// A batch concatenates the atoms of N systems of uneven size in HBM.  One MD
+// step runs the same kernels as a single system, over all systems at once:
+// the GB passes walk a flat list of tile pairs of every system (k_gb.hip), the
+// bonded kernel covers the concatenated term lists, and the integrator reads
+// each atom's system temperature.  Per-system energies come from segmented
+// FP64 reductions.  The step is recorded once into a hipGraph and replayed, with
+// the step number kept on the device, so a run of n steps is n graph launches
+// and no host round trip.
+//
+// REX_Local (remd.cpp:65-254) runs its replicas one after another on one
+// WorkSpace; here they are the systems of one batch.  After each segment the
+// host applies PD's alternating neighbour exchanges with the Metropolis test of
+// exchangeCriterion (remd.cpp:349-357) and the velocity rescale
+// sqrt(T_hi/T_lo) (remd.cpp:206-214), swapping the replicas' structures.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace pdgpu {
+
+namespace {
+
+constexpr int kStride = 12;  // per-system energy slots, kcal/mol except [8]
+// [0] GB cross  [1] vdW  [2] elec  [3] -  [4] GB self  [5] bond  [6] angle
+// [7] torsion  [8] kinetic (J)  [9] epot (J)
+
+__global__ void k_zero3n(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+    c[i] = 0;
+  }
+}
+
+__global__ void k_seg_sum(const double *__restrict__ v, const int *__restrict__ beg, const int *__restrict__ end,
+                          double *__restrict__ out, int stride, int off) {
+  __shared__ double s[256];
+  const int sysi = blockIdx.x;
+  double a = 0.0;
+  for (int k = beg[sysi] + (int)threadIdx.x; k < end[sysi]; k += 256) a += v[k];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)sysi * stride + off] = s[0];
+}
+
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u53d(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+struct BatchLangevin {
+  double t, gamma, gammat;
+  double c0, c1, c2, varr, varv, crv;
+  int total_steps;
+  int vv;           // velocity Verlet instead of Langevin
+  uint32_t k0, k1;
+  uint32_t stream;  // noise stream of system 0; system s uses stream + s
+  double conv;
+};
+
+// One integrator step for every atom of every system (md.cpp:805-855 / 923-1059);
+// the step number lives on the device so the launch can be replayed in a graph.
+__global__ __launch_bounds__(256) void k_batch_integrate(
+    int n, BatchLangevin L, const int *__restrict__ step_ptr, const int *__restrict__ atom_sys,
+    const int *__restrict__ sys_first, const double *__restrict__ temps, const double *__restrict__ mass,
+    const long long *__restrict__ fxp,
+    const long long *__restrict__ fyp, const long long *__restrict__ fzp, double *__restrict__ x,
+    double *__restrict__ y, double *__restrict__ z, double *__restrict__ vx, double *__restrict__ vy,
+    double *__restrict__ vz, double *__restrict__ ke) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int step = *step_ptr;
+  const double f[3] = {(double)fxp[i] * L.conv, (double)fyp[i] * L.conv, (double)fzp[i] * L.conv};
+  const double m = mass[i];
+  const double t = L.t;
+  const double ia = 1 / Consts::Angstrom;
+  double v[3] = {vx[i], vy[i], vz[i]};
+  double p[3] = {x[i], y[i], z[i]};
+  if (L.vv) {
+    const double tinvmass = 0.5 * t / m;
+    for (int d = 0; d < 3; d++) v[d] += f[d] * tinvmass;
+    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    ke[i] = 0.5 * m * (mag * mag);
+    for (int d = 0; d < 3; d++) {
+      const double dv = f[d] * tinvmass;
+      v[d] += dv;
+      double dr = t * (v[d] + dv);
+      dr *= ia;
+      p[d] += dr;
+      v[d] += dv;
+    }
+  } else {
+    const double tinvmass = t / m;
+    if (step > 0)
+      for (int d = 0; d < 3; d++) v[d] += (1.0 - L.c0 / L.c1) * f[d] * tinvmass / L.gammat;
+    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    ke[i] = 0.5 * m * (mag * mag);
+    if (step < (L.total_steps - 1)) {
+      const double kTdivm = Consts::kB * temps[atom_sys[i]] / m;
+      const double sigmav = sqrt(kTdivm * L.varv);
+      const double sigmar = sqrt(kTdivm * L.varr) / L.gamma;
+      double zz[6];
+      const double TwoPI = 2.0 * Consts::PI;
+      const int sys = atom_sys[i];
+      for (int k = 0; k < 3; k++) {
+        uint32_t c[4] = {(uint32_t)(i - sys_first[sys]), (uint32_t)step, L.stream + (uint32_t)sys, (uint32_t)k};
+        philox10(c, L.k0, L.k1);
+        const double u1 = u53d(c[0], c[1]), u2 = u53d(c[2], c[3]);
+        const double l = sqrt(-2.0 * log(u1));
+        zz[2 * k] = l * cos((-TwoPI) * u2);
+        zz[2 * k + 1] = l * sin((-TwoPI) * u2);
+      }
+      const double n1[3] = {zz[0], zz[4], zz[3]};
+      const double n2[3] = {zz[2], zz[1], zz[5]};
+      const double s = sqrt(1.0 - L.crv * L.crv);
+      for (int d = 0; d < 3; d++) {
+        const double delv = sigmav * (L.crv * n1[d] + s * n2[d]);
+        const double delr = sigmar * n1[d];
+        double dr = t * (L.c1 * v[d] + L.c2 * f[d] * tinvmass) + delr;
+        dr *= ia;
+        const double dv = (L.c0 * L.c2) * f[d] * tinvmass / L.c1 + delv;
+        p[d] += dr;
+        v[d] *= L.c0;
+        v[d] += dv;
+      }
+    }
+  }
+  x[i] = p[0];
+  y[i] = p[1];
+  z[i] = p[2];
+  vx[i] = v[0];
+  vy[i] = v[1];
+  vz[i] = v[2];
+}
+
+// Per-system epot/ekin of this step into the trace; advances the step counter.
+__global__ void k_batch_record(int nsys, double *__restrict__ e_sys, double kcal_to_J, int *__restrict__ counters,
+                               double *__restrict__ trace) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int local = counters[1];
+  if (s < nsys) {
+    double *e = e_sys + (size_t)s * kStride;
+    const double ep = (e[0] + e[1] + e[2] + e[4] + e[5] + e[6] + e[7]) * kcal_to_J;
+    e[9] = ep;
+    trace[((size_t)local * nsys + s) * 2 + 0] = ep;
+    trace[((size_t)local * nsys + s) * 2 + 1] = e[8];
+  }
+}
+
+__global__ void k_advance(int *__restrict__ counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[0] += 1;  // protocol step (Langevin first/last-step guards, noise counter)
+    counters[1] += 1;  // trace row
+  }
+}
+
+__global__ void k_set_counters(int *__restrict__ counters, int step, int local) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[0] = step;
+    counters[1] = local;
+  }
+}
+
+// Exchanges the structures of replica pairs (a, b): positions swap, velocities
+// swap and are rescaled: a receives b's velocities / f, b receives a's * f with
+// f = sqrt(T_b / T_a) (remd.cpp:190-214).
+__global__ void k_swap_pairs(int npairs, const int *__restrict__ pa, const int *__restrict__ pb,
+                             const double *__restrict__ factor, const int *__restrict__ first, int count,
+                             double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                             double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz) {
+  const int k = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npairs || i >= count) return;
+  const int a = first[pa[k]] + i, b = first[pb[k]] + i;
+  const double f = factor[k];
+  double t;
+  t = x[a]; x[a] = x[b]; x[b] = t;
+  t = y[a]; y[a] = y[b]; y[b] = t;
+  t = z[a]; z[a] = z[b]; z[b] = t;
+  const double va[3] = {vx[a], vy[a], vz[a]}, vb[3] = {vx[b], vy[b], vz[b]};
+  vx[a] = vb[0] / f;
+  vy[a] = vb[1] / f;
+  vz[a] = vb[2] / f;
+  vx[b] = va[0] * f;
+  vy[b] = va[1] * f;
+  vz[b] = va[2] * f;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  int nsys = 0, natoms = 0;
+  std::vector<int> first, count;
+  DBuf<double> x, y, z, vx, vy, vz, mass;
+  DBuf<long long> fx, fy, fz;
+  DBuf<int> atom_sys, sys_first;
+  std::unique_ptr<GBState> gb;
+  bool have_gb = false;
+  // bonded terms, global atom indices, numbered bonds | angles | torsions
+  DBuf<int> bi, bj, ai, aa, aj, ti, ta, tb, tj, tterms;
+  DBuf<double> bl, bk, at0, ak, tvn, tnn, tgamma;
+  int nb = 0, na = 0, nt = 0;
+  DBuf<int> seg_beg[3], seg_end[3];  // per-system term ranges (bond, angle, torsion numbering)
+  DBuf<double> eterm, epart_dummy, ke;
+  DBuf<int> atom_beg, atom_end;
+  DBuf<double> e_sys, temps, trace;
+  DBuf<int> counters;
+  std::vector<double> h_temps;
+  // graph of one step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  BatchLangevin graph_args{};
+  uint32_t graph_components = 0;
+  int graph_trace_rows = 0;
+  // REMD swap scratch
+  DBuf<int> swap_a, swap_b;
+  DBuf<double> swap_f;
+
+  ~Batch() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+  void sync() { PDGPU_HIP(hipStreamSynchronize(stream)); }
+
+  void record_step(uint32_t comps, const BatchLangevin &L) {
+    const int T = 256;
+    hipLaunchKernelGGL(k_zero3n, dim3((natoms + T - 1) / T), dim3(T), 0, stream, fx.p, fy.p, fz.p, natoms);
+    if ((comps & PDGPU_GB) && have_gb) gb->run(stream, x.p, y.p, z.p, fx.p, fy.p, fz.p, e_sys.p, kStride, 0, nullptr);
+    if ((comps & PDGPU_BONDED) && nb + na + nt > 0) {
+      BondedArrays B{nb, na, nt, bi.p, bj.p, bl.p, bk.p, ai.p, aa.p, aj.p, at0.p, ak.p,
+                     ti.p, ta.p, tb.p, tj.p, tterms.p, tvn.p, tnn.p, tgamma.p};
+      launch_bonded_kernel(stream, B, x.p, y.p, z.p, fx.p, fy.p, fz.p, epart_dummy.p, eterm.p);
+      for (int c = 0; c < 3; c++)
+        hipLaunchKernelGGL(k_seg_sum, dim3(nsys), dim3(256), 0, stream, eterm.p, seg_beg[c].p, seg_end[c].p,
+                           e_sys.p, kStride, 5 + c);
+    }
+    hipLaunchKernelGGL(k_batch_integrate, dim3((natoms + T - 1) / T), dim3(T), 0, stream, natoms, L, counters.p,
+                       atom_sys.p, sys_first.p, temps.p, mass.p, fx.p, fy.p, fz.p, x.p, y.p, z.p, vx.p, vy.p, vz.p,
+                       ke.p);
+    hipLaunchKernelGGL(k_seg_sum, dim3(nsys), dim3(256), 0, stream, ke.p, atom_beg.p, atom_end.p, e_sys.p, kStride,
+                       8);
+    hipLaunchKernelGGL(k_batch_record, dim3((nsys + 63) / 64), dim3(64), 0, stream, nsys, e_sys.p,
+                       Consts::kcal_to_J(), counters.p, trace.p);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, stream, counters.p);
+  }
+
+  void zero_energies() {
+    std::vector<double> z0((size_t)nsys * kStride, 0.0);
+    e_sys.upload(z0, stream);
+  }
+
+  // n steps of the batch MD; step numbers step_offset .. step_offset+n-1 of a
+  // protocol run of p.total_steps steps.
+  void md(const pdgpu_md_params &p, int n, double *ep_trace, double *ek_trace) {
+    if (n <= 0) return;
+    BatchLangevin L{};
+    L.t = p.timestep;
+    L.gamma = p.fric_coeff;
+    L.gammat = p.fric_coeff * p.timestep;
+    L.vv = (p.integrator == PDGPU_VELOCITY_VERLET || L.gammat < 0.00001) ? 1 : 0;
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    if (!L.vv) langevin_coefficients(L.gammat, c);
+    L.c0 = c[0];
+    L.c1 = c[1];
+    L.c2 = c[2];
+    L.varr = c[3];
+    L.varv = c[4];
+    L.crv = c[5];
+    L.total_steps = p.total_steps;
+    L.k0 = (uint32_t)p.seed;
+    L.k1 = (uint32_t)(p.seed >> 32);
+    L.stream = p.noise_stream;
+    L.conv = Consts::kcalA_to_Jm() / kFixScale;
+    const int rows = n;
+    const bool rebuild = !graph_exec || graph_components != p.components || graph_trace_rows < rows ||
+                         std::memcmp(&graph_args, &L, sizeof(L)) != 0;
+    if (rebuild) {
+      if (graph_exec) PDGPU_HIP(hipGraphExecDestroy(graph_exec));
+      if (graph) PDGPU_HIP(hipGraphDestroy(graph));
+      graph_exec = nullptr;
+      graph = nullptr;
+      trace.resize((size_t)rows * nsys * 2);
+      zero_energies();
+      sync();
+      PDGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      record_step(p.components, L);
+      PDGPU_HIP(hipStreamEndCapture(stream, &graph));
+      PDGPU_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      graph_args = L;
+      graph_components = p.components;
+      graph_trace_rows = rows;
+    }
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, stream, counters.p, p.step_offset, 0);
+    for (int s = 0; s < n; s++) PDGPU_HIP(hipGraphLaunch(graph_exec, stream));
+    std::vector<double> h((size_t)n * nsys * 2);
+    trace.download(h.data(), h.size(), stream);
+    sync();
+    bool nan = false;
+    for (int s = 0; s < n; s++)
+      for (int k = 0; k < nsys; k++) {
+        const double e0 = h[((size_t)s * nsys + k) * 2], e1 = h[((size_t)s * nsys + k) * 2 + 1];
+        if (ep_trace) ep_trace[(size_t)s * nsys + k] = e0;
+        if (ek_trace) ek_trace[(size_t)s * nsys + k] = e1;
+        if (std::isnan(e0)) nan = true;
+      }
+    if (nan) throw EngineError(PDGPU_ERR_UNSTABLE, "Simulation unstable or division by zero");
+  }
+};
+
+}  // namespace pdgpu
+
+using pdgpu::Batch;
+using pdgpu::Engine;
+using pdgpu::EngineError;
+
+namespace {
+
+template <typename F>
+int bguard(pdgpu_batch *h, F &&fn) {
+  Batch *b = reinterpret_cast<Batch *>(h);
+  if (!b) return PDGPU_ERR_ARG;
+  try {
+    PDGPU_HIP(hipSetDevice(b->device));
+    fn(*b);
+    return PDGPU_OK;
+  } catch (const EngineError &x) {
+    b->last_error = x.what();
+    return x.code;
+  } catch (const std::exception &x) {
+    b->last_error = x.what();
+    return PDGPU_ERR_ARG;
+  }
+}
+
+template <typename T>
+std::vector<T> fetch(const pdgpu::DBuf<T> &d, size_t n, hipStream_t s) {
+  std::vector<T> h(n);
+  if (n) d.download(h.data(), n, s);
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_batch **out) {
+  if (!out || n_systems <= 0 || !systems) return PDGPU_ERR_ARG;
+  *out = nullptr;
+  auto B = std::make_unique<Batch>();
+  try {
+    Engine *e0 = reinterpret_cast<Engine *>(systems[0]);
+    if (!e0) return PDGPU_ERR_ARG;
+    B->device = e0->device;
+    PDGPU_HIP(hipSetDevice(B->device));
+    PDGPU_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    B->nsys = n_systems;
+    std::vector<double> X, Y, Z, VX, VY, VZ, M;
+    std::vector<int> asys;
+    std::vector<int> Bi, Bj, Ai, Aa, Aj, Ti, Ta, Tb, Tj, Tt;
+    std::vector<double> Bl, Bk, At0, Ak, Tvn, Tnn, Tg;
+    std::vector<int> sb[3], se[3];
+    std::vector<float> q, ljr, lje, gbr, vj;
+    std::vector<double> t123;
+    auto gb = std::make_unique<pdgpu::GBState>();
+    bool all_gb = true;
+    for (int s = 0; s < n_systems; s++) {
+      Engine *E = reinterpret_cast<Engine *>(systems[s]);
+      if (!E || !E->have_positions || E->mass.n < (size_t)E->n) {
+        if (E) E->last_error = "batch member needs atom parameters and positions";
+        return PDGPU_ERR_ARG;
+      }
+      const int off = B->natoms, n = E->n;
+      B->first.push_back(off);
+      B->count.push_back(n);
+      auto app = [&](std::vector<double> &dst, const pdgpu::DBuf<double> &src) {
+        auto h = fetch(src, n, E->stream);
+        dst.insert(dst.end(), h.begin(), h.end());
+      };
+      app(X, E->x); app(Y, E->y); app(Z, E->z); app(VX, E->vx); app(VY, E->vy); app(VZ, E->vz);
+      app(M, E->mass);
+      asys.insert(asys.end(), n, s);
+      // bonded terms, shifted to global indices
+      auto shifted = [&](const pdgpu::DBuf<int> &src, int cnt, std::vector<int> &dst) {
+        auto h = fetch(src, cnt, E->stream);
+        for (int v : h) dst.push_back(v + off);
+      };
+      auto copyd = [&](const pdgpu::DBuf<double> &src, size_t cnt, std::vector<double> &dst) {
+        auto h = fetch(src, cnt, E->stream);
+        dst.insert(dst.end(), h.begin(), h.end());
+      };
+      sb[0].push_back((int)Bi.size());
+      if (E->nbonds) {
+        shifted(E->b_i, E->nbonds, Bi); shifted(E->b_j, E->nbonds, Bj);
+        copyd(E->b_l, E->nbonds, Bl); copyd(E->b_k, E->nbonds, Bk);
+      }
+      se[0].push_back((int)Bi.size());
+      sb[1].push_back((int)Ai.size());
+      if (E->nangles) {
+        shifted(E->a_i, E->nangles, Ai); shifted(E->a_a, E->nangles, Aa); shifted(E->a_j, E->nangles, Aj);
+        copyd(E->a_t0, E->nangles, At0); copyd(E->a_k, E->nangles, Ak);
+      }
+      se[1].push_back((int)Ai.size());
+      sb[2].push_back((int)Ti.size());
+      if (E->ntors) {
+        shifted(E->t_i, E->ntors, Ti); shifted(E->t_a, E->ntors, Ta); shifted(E->t_b, E->ntors, Tb);
+        shifted(E->t_j, E->ntors, Tj);
+        auto tt = fetch(E->t_terms, E->ntors, E->stream);
+        Tt.insert(Tt.end(), tt.begin(), tt.end());
+        copyd(E->t_vn, 4 * (size_t)E->ntors, Tvn); copyd(E->t_nn, 4 * (size_t)E->ntors, Tnn);
+        copyd(E->t_gamma, 4 * (size_t)E->ntors, Tg);
+      }
+      se[2].push_back((int)Ti.size());
+      if (E->have_gb) {
+        if (s == 0) gb->set_params(E->gb->prm);
+        else if (std::memcmp(&E->gb->prm, &gb->prm, sizeof(pdgpu_gb_params)) != 0) {
+          E->last_error = "batch members must share GB parameters";
+          return PDGPU_ERR_ARG;
+        }
+        gb->add_system(off, n, E->mid, E->h_band, E->h_sp_i, E->h_sp_j, E->h_sp_order);
+        auto f = [&](std::vector<float> &dst, const pdgpu::DBuf<float> &src) {
+          auto h = fetch(src, n, E->stream);
+          dst.insert(dst.end(), h.begin(), h.end());
+        };
+        f(q, E->gb->q); f(ljr, E->gb->ljr); f(lje, E->gb->lje); f(gbr, E->gb->gbrad); f(vj, E->gb->Vj);
+        copyd(E->gb->terms123, n, t123);
+      } else {
+        all_gb = false;
+      }
+      E->sync();
+      B->natoms += n;
+    }
+    Batch &b = *B;
+    const hipStream_t st = b.stream;
+    b.x.upload(X, st); b.y.upload(Y, st); b.z.upload(Z, st);
+    b.vx.upload(VX, st); b.vy.upload(VY, st); b.vz.upload(VZ, st);
+    b.mass.upload(M, st);
+    b.atom_sys.upload(asys, st);
+    b.fx.resize(b.natoms); b.fy.resize(b.natoms); b.fz.resize(b.natoms);
+    b.ke.resize(b.natoms);
+    b.nb = (int)Bi.size(); b.na = (int)Ai.size(); b.nt = (int)Ti.size();
+    auto upi = [&](pdgpu::DBuf<int> &d, const std::vector<int> &v) { d.upload(v.empty() ? std::vector<int>{0} : v, st); };
+    auto upd = [&](pdgpu::DBuf<double> &d, const std::vector<double> &v) {
+      d.upload(v.empty() ? std::vector<double>{0.0} : v, st);
+    };
+    upi(b.bi, Bi); upi(b.bj, Bj); upd(b.bl, Bl); upd(b.bk, Bk);
+    upi(b.ai, Ai); upi(b.aa, Aa); upi(b.aj, Aj); upd(b.at0, At0); upd(b.ak, Ak);
+    upi(b.ti, Ti); upi(b.ta, Ta); upi(b.tb, Tb); upi(b.tj, Tj); upi(b.tterms, Tt);
+    upd(b.tvn, Tvn); upd(b.tnn, Tnn); upd(b.tgamma, Tg);
+    // term numbering in the kernel: bonds, then angles, then torsions
+    for (int s = 0; s < n_systems; s++) {
+      sb[1][s] += b.nb; se[1][s] += b.nb;
+      sb[2][s] += b.nb + b.na; se[2][s] += b.nb + b.na;
+    }
+    for (int c = 0; c < 3; c++) {
+      b.seg_beg[c].upload(sb[c], st);
+      b.seg_end[c].upload(se[c], st);
+    }
+    b.eterm.resize(std::max(1, b.nb + b.na + b.nt));
+    b.epart_dummy.resize(4 * (size_t)std::max(1, (b.nb + b.na + b.nt + 255) / 256));
+    std::vector<int> ab(b.first), ae(n_systems);
+    for (int s = 0; s < n_systems; s++) ae[s] = b.first[s] + b.count[s];
+    b.atom_beg.upload(ab, st);
+    b.atom_end.upload(ae, st);
+    b.sys_first.upload(b.first, st);
+    b.e_sys.resize((size_t)n_systems * pdgpu::kStride);
+    b.h_temps.assign(n_systems, 300.0);
+    b.temps.upload(b.h_temps, st);
+    b.counters.resize(2);
+    if (all_gb) {
+      gb->upload(st);
+      gb->q.upload(q, st); gb->ljr.upload(ljr, st); gb->lje.upload(lje, st);
+      gb->gbrad.upload(gbr, st); gb->Vj.upload(vj, st); gb->terms123.upload(t123, st);
+      b.gb = std::move(gb);
+      b.have_gb = true;
+    }
+    b.zero_energies();
+    b.sync();
+    *out = reinterpret_cast<pdgpu_batch *>(B.release());
+    return PDGPU_OK;
+  } catch (const EngineError &x) {
+    return x.code;
+  } catch (...) {
+    return PDGPU_ERR_HIP;
+  }
+}
+
+int pdgpu_batch_destroy(pdgpu_batch *h) {
+  Batch *b = reinterpret_cast<Batch *>(h);
+  if (!b) return PDGPU_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  delete b;
+  return PDGPU_OK;
+}
+
+const char *pdgpu_batch_last_error(const pdgpu_batch *h) {
+  const Batch *b = reinterpret_cast<const Batch *>(h);
+  return b ? b->last_error.c_str() : "null batch handle";
+}
+
+int pdgpu_batch_set_temperatures(pdgpu_batch *h, const double *temps) {
+  return bguard(h, [&](Batch &b) {
+    if (!temps) throw EngineError(PDGPU_ERR_ARG, "null temperatures");
+    for (int s = 0; s < b.nsys; s++)
+      if (!(temps[s] > 0.0)) throw EngineError(PDGPU_ERR_ARG, "temperatures must be positive");
+    b.h_temps.assign(temps, temps + b.nsys);
+    b.temps.upload(b.h_temps, b.stream);
+    b.sync();
+  });
+}
+
+static int batch_state(pdgpu_batch *h, int32_t sys, double *xyz, size_t stride, int which, bool set) {
+  return bguard(h, [&](Batch &b) {
+    if (sys < 0 || sys >= b.nsys || !xyz || stride < 3 * sizeof(double))
+      throw EngineError(PDGPU_ERR_ARG, "bad system index or array");
+    const int off = b.first[sys], n = b.count[sys];
+    pdgpu::DBuf<double> *d[3] = {which ? &b.vx : &b.x, which ? &b.vy : &b.y, which ? &b.vz : &b.z};
+    std::vector<double> a(n), c(n), e(n);
+    std::vector<double> *v[3] = {&a, &c, &e};
+    if (set) {
+      for (int i = 0; i < n; i++) {
+        const double *p = reinterpret_cast<const double *>(reinterpret_cast<const char *>(xyz) + (size_t)i * stride);
+        a[i] = p[0];
+        c[i] = p[1];
+        e[i] = p[2];
+      }
+      for (int k = 0; k < 3; k++)
+        PDGPU_HIP(hipMemcpyAsync(d[k]->p + off, v[k]->data(), n * sizeof(double), hipMemcpyHostToDevice, b.stream));
+      b.sync();
+    } else {
+      for (int k = 0; k < 3; k++)
+        PDGPU_HIP(hipMemcpyAsync(v[k]->data(), d[k]->p + off, n * sizeof(double), hipMemcpyDeviceToHost, b.stream));
+      b.sync();
+      for (int i = 0; i < n; i++) {
+        double *p = reinterpret_cast<double *>(reinterpret_cast<char *>(xyz) + (size_t)i * stride);
+        p[0] = a[i];
+        p[1] = c[i];
+        p[2] = e[i];
+      }
+    }
+  });
+}
+
+int pdgpu_batch_get_positions(pdgpu_batch *h, int32_t sys, double *xyz, size_t stride_bytes) {
+  return batch_state(h, sys, xyz, stride_bytes, 0, false);
+}
+int pdgpu_batch_set_positions(pdgpu_batch *h, int32_t sys, const double *xyz, size_t stride_bytes) {
+  return batch_state(h, sys, const_cast<double *>(xyz), stride_bytes, 0, true);
+}
+int pdgpu_batch_get_velocities(pdgpu_batch *h, int32_t sys, double *xyz, size_t stride_bytes) {
+  return batch_state(h, sys, xyz, stride_bytes, 1, false);
+}
+int pdgpu_batch_set_velocities(pdgpu_batch *h, int32_t sys, const double *xyz, size_t stride_bytes) {
+  return batch_state(h, sys, const_cast<double *>(xyz), stride_bytes, 1, true);
+}
+
+int pdgpu_batch_compute(pdgpu_batch *h, uint32_t components, double *f_accum, pdgpu_energies *energies) {
+  return bguard(h, [&](Batch &b) {
+    const int T = 256;
+    b.zero_energies();
+    hipLaunchKernelGGL(pdgpu::k_zero3n, dim3((b.natoms + T - 1) / T), dim3(T), 0, b.stream, b.fx.p, b.fy.p, b.fz.p,
+                       b.natoms);
+    if ((components & PDGPU_GB) && b.have_gb)
+      b.gb->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.e_sys.p, pdgpu::kStride, 0, nullptr);
+    if ((components & PDGPU_BONDED) && b.nb + b.na + b.nt > 0) {
+      pdgpu::BondedArrays B{b.nb, b.na, b.nt, b.bi.p, b.bj.p, b.bl.p, b.bk.p, b.ai.p, b.aa.p, b.aj.p, b.at0.p,
+                            b.ak.p, b.ti.p, b.ta.p, b.tb.p, b.tj.p, b.tterms.p, b.tvn.p, b.tnn.p, b.tgamma.p};
+      pdgpu::launch_bonded_kernel(b.stream, B, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.epart_dummy.p,
+                                  b.eterm.p);
+      for (int c = 0; c < 3; c++)
+        hipLaunchKernelGGL(pdgpu::k_seg_sum, dim3(b.nsys), dim3(256), 0, b.stream, b.eterm.p, b.seg_beg[c].p,
+                           b.seg_end[c].p, b.e_sys.p, pdgpu::kStride, 5 + c);
+    }
+    PDGPU_HIP(hipGetLastError());
+    std::vector<double> e((size_t)b.nsys * pdgpu::kStride);
+    b.e_sys.download(e.data(), e.size(), b.stream);
+    std::vector<long long> fa, fb, fc;
+    if (f_accum) {
+      fa = fetch(b.fx, b.natoms, b.stream);
+      fb = fetch(b.fy, b.natoms, b.stream);
+      fc = fetch(b.fz, b.natoms, b.stream);
+    }
+    b.sync();
+    const double kj = pdgpu::Consts::kcal_to_J();
+    if (energies)
+      for (int s = 0; s < b.nsys; s++) {
+        const double *v = &e[(size_t)s * pdgpu::kStride];
+        pdgpu_energies o{};
+        o.epot_pol_cross = v[0] * kj;
+        o.epot_vdw = v[1] * kj;
+        o.epot_elec = v[2] * kj;
+        o.epot_pol_self = v[4] * kj;
+        o.epot_pol = (v[0] + v[4]) * kj;
+        o.epot_bond = v[5] * kj;
+        o.epot_angle = v[6] * kj;
+        o.epot_torsion = v[7] * kj;
+        o.epot = (v[0] + v[1] + v[2] + v[4] + v[5] + v[6] + v[7]) * kj;
+        energies[s] = o;
+      }
+    if (f_accum) {
+      const double conv = pdgpu::Consts::kcalA_to_Jm() / pdgpu::kFixScale;
+      for (int i = 0; i < b.natoms; i++) {
+        f_accum[3 * i + 0] += (double)fa[i] * conv;
+        f_accum[3 * i + 1] += (double)fb[i] * conv;
+        f_accum[3 * i + 2] += (double)fc[i] * conv;
+      }
+    }
+  });
+}
+
+int pdgpu_batch_md(pdgpu_batch *h, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
+                   double *ekin_trace) {
+  return bguard(h, [&](Batch &b) {
+    if (!p || n_steps < 0) throw EngineError(PDGPU_ERR_ARG, "bad md arguments");
+    if (p->integrator != PDGPU_VELOCITY_VERLET && p->integrator != PDGPU_LANGEVIN)
+      throw EngineError(PDGPU_ERR_ARG, "unknown integrator");
+    if (p->timestep <= 0) throw EngineError(PDGPU_ERR_ARG, "timestep must be positive");
+    b.md(*p, n_steps, epot_trace, ekin_trace);
+  });
+}
+
+int pdgpu_batch_remd(pdgpu_batch *h, const pdgpu_md_params *p, int32_t n_rounds, uint64_t seed,
+                     int32_t *accepted, int32_t *attempted, double *final_epot) {
+  return bguard(h, [&](Batch &b) {
+    if (!p || n_rounds < 0) throw EngineError(PDGPU_ERR_ARG, "bad remd arguments");
+    for (int s = 1; s < b.nsys; s++)
+      if (b.count[s] != b.count[0]) throw EngineError(PDGPU_ERR_ARG, "REMD replicas must have equal atom counts");
+    const int seg = p->total_steps;
+    if (seg <= 0) throw EngineError(PDGPU_ERR_ARG, "total_steps is the exchange interval and must be > 0");
+    std::vector<double> ep((size_t)seg * b.nsys);
+    for (int k = 0; k < b.nsys; k++) {
+      if (accepted) accepted[k] = 0;
+      if (attempted) attempted[k] = 0;
+    }
+    b.swap_a.resize(b.nsys);
+    b.swap_b.resize(b.nsys);
+    b.swap_f.resize(b.nsys);
+    const double kB = pdgpu::Consts::kB;
+    for (int round = 0; round < n_rounds; round++) {
+      pdgpu_md_params q = *p;
+      q.step_offset = 0;  // every replica run starts at Step 0 (remd.cpp:122-123)
+      q.seed = p->seed + (uint64_t)round * 0x9E3779B97F4A7C15ull;
+      b.md(q, seg, ep.data(), nullptr);
+      const double *last = &ep[(size_t)(seg - 1) * b.nsys];
+      std::vector<double> e(last, last + b.nsys);
+      std::vector<int> pa, pb;
+      std::vector<double> pf;
+      for (int r = round % 2; r < b.nsys - 1; r += 2) {
+        // exchangeCriterion (remd.cpp:349-357) with a Philox uniform for frand()
+        const double delta = (e[r] - e[r + 1]) * (1.0 / (kB * b.h_temps[r + 1]) - 1.0 / (kB * b.h_temps[r]));
+        uint32_t c[4] = {(uint32_t)r, (uint32_t)round, 0x52454D44u, 0u};
+        uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+        for (int it = 0; it < 10; it++) {
+          const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+          const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+          c[1] = (uint32_t)p1;
+          c[3] = (uint32_t)p0;
+          c[0] = n0;
+          c[2] = n2;
+          k0 += 0x9E3779B9u;
+          k1 += 0xBB67AE85u;
+        }
+        const uint64_t m = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+        const double u = ((double)m + 1.0) * (1.0 / 9007199254740992.0);
+        if (attempted) attempted[r]++;
+        if (delta <= 0 || std::exp(-delta) > u) {
+          if (accepted) accepted[r]++;
+          pa.push_back(r);
+          pb.push_back(r + 1);
+          pf.push_back(std::sqrt(b.h_temps[r + 1] / b.h_temps[r]));
+        }
+      }
+      if (!pa.empty()) {
+        PDGPU_HIP(hipMemcpyAsync(b.swap_a.p, pa.data(), pa.size() * sizeof(int), hipMemcpyHostToDevice, b.stream));
+        PDGPU_HIP(hipMemcpyAsync(b.swap_b.p, pb.data(), pb.size() * sizeof(int), hipMemcpyHostToDevice, b.stream));
+        PDGPU_HIP(hipMemcpyAsync(b.swap_f.p, pf.data(), pf.size() * sizeof(double), hipMemcpyHostToDevice, b.stream));
+        const int cnt = b.count[0];
+        hipLaunchKernelGGL(pdgpu::k_swap_pairs, dim3((cnt + 255) / 256, (unsigned)pa.size()), dim3(256), 0, b.stream,
+                           (int)pa.size(), b.swap_a.p, b.swap_b.p, b.swap_f.p, b.sys_first.p, cnt, b.x.p, b.y.p,
+                           b.z.p, b.vx.p, b.vy.p, b.vz.p);
+        PDGPU_HIP(hipGetLastError());
+        b.sync();
+      }
+      if (final_epot) std::copy(e.begin(), e.end(), final_epot);
+    }
+  });
+}
+
+}  // extern "C"

@@ -32,6 +32,7 @@ __device__ void block_partial(double v0, double v1, double v2, double v3, double
   if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// eterm (optional): the energy of each term, for batches that reduce it per system.
 __global__ __launch_bounds__(256) void k_bonded(
     int nb, const int *__restrict__ bi, const int *__restrict__ bj, const double *__restrict__ bl,
     const double *__restrict__ bk, int na, const int *__restrict__ ai, const int *__restrict__ aa,
@@ -40,7 +41,8 @@ __global__ __launch_bounds__(256) void k_bonded(
     const int *__restrict__ tj, const int *__restrict__ tterms, const double *__restrict__ tvn,
     const double *__restrict__ tnn, const double *__restrict__ tgamma, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, long long *__restrict__ fx,
-    long long *__restrict__ fy, long long *__restrict__ fz, double *__restrict__ partial) {
+    long long *__restrict__ fy, long long *__restrict__ fz, double *__restrict__ partial,
+    double *__restrict__ eterm) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   double eb = 0.0, ea = 0.0, et = 0.0, vir = 0.0;
   if (t < nb) {
@@ -176,23 +178,30 @@ __global__ __launch_bounds__(256) void k_bonded(
     add3(fx, fy, fz, i2, fjx - fabx, fjy - faby, fjz - fabz);
     add3(fx, fy, fz, i3, -fjx, -fjy, -fjz);
   }
+  if (eterm && t < nb + na + nt) eterm[t] = eb + ea + et;
   block_partial(eb, ea, et, vir, partial);
 }
 
 }  // namespace
 
-// Reduction kernel shared with the other components (k_nonbonded.hip).
-void reduce4(hipStream_t s, const double *part, int count, double *out);
+void launch_bonded_kernel(hipStream_t s, const BondedArrays &B, const double *x, const double *y,
+                          const double *z, long long *fx, long long *fy, long long *fz, double *partial,
+                          double *eterm) {
+  const int total = B.nb + B.na + B.nt;
+  const int grid = std::max(1, (total + 255) / 256);
+  hipLaunchKernelGGL(k_bonded, dim3(grid), dim3(256), 0, s, B.nb, B.bi, B.bj, B.bl, B.bk, B.na, B.ai, B.aa, B.aj,
+                     B.at0, B.ak, B.nt, B.ti, B.ta, B.tb, B.tj, B.tterms, B.tvn, B.tnn, B.tgamma, x, y, z, fx, fy, fz,
+                     partial, eterm);
+}
 
 void Engine::launch_bonded(double *energy_out) {
   const int total = nbonds + nangles + ntors;
-  const int T = 256;
-  const int grid = std::max(1, (total + T - 1) / T);
+  const int grid = std::max(1, (total + 255) / 256);
   DBuf<double> &part = e_partial_bonded;
   part.resize((size_t)grid * 4);
-  hipLaunchKernelGGL(k_bonded, dim3(grid), dim3(T), 0, stream, nbonds, b_i.p, b_j.p, b_l.p, b_k.p, nangles, a_i.p,
-                     a_a.p, a_j.p, a_t0.p, a_k.p, ntors, t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p,
-                     t_gamma.p, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, part.p);
+  BondedArrays B{nbonds, nangles, ntors, b_i.p, b_j.p, b_l.p, b_k.p, a_i.p, a_a.p, a_j.p, a_t0.p, a_k.p,
+                 t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p, t_gamma.p};
+  launch_bonded_kernel(stream, B, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, part.p, nullptr);
   PDGPU_HIP(hipGetLastError());
   reduce4(stream, part.p, grid, energy_out);
 }

@@ -341,7 +341,7 @@ void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
     L.move = step < (p.total_steps - 1) ? 1 : 0;
     L.k0 = (uint32_t)p.seed;
     L.k1 = (uint32_t)(p.seed >> 32);
-    L.stream = 0;
+    L.stream = p.noise_stream;
     L.step = (uint32_t)step;
     L.conv = conv;
     hipLaunchKernelGGL(k_langevin, dim3(grid), dim3(T), 0, stream, n, L, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az,

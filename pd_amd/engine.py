@@ -35,6 +35,9 @@ EXPORTS = (
     "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_build_list", "pdgpu_full_update_count",
     "pdgpu_get_pair_count", "pdgpu_get_pairs", "pdgpu_compute", "pdgpu_get_born_radii", "pdgpu_md",
     "pdgpu_set_timing", "pdgpu_kernel_time",
+    "pdgpu_batch_create", "pdgpu_batch_destroy", "pdgpu_batch_last_error", "pdgpu_batch_set_temperatures",
+    "pdgpu_batch_get_positions", "pdgpu_batch_set_positions", "pdgpu_batch_get_velocities",
+    "pdgpu_batch_set_velocities", "pdgpu_batch_compute", "pdgpu_batch_md", "pdgpu_batch_remd",
 )
 
 
@@ -76,7 +79,8 @@ class GBParams(C.Structure):
 class MDParams(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("components", C.c_uint32), ("timestep", C.c_double),
                 ("temperature", C.c_double), ("fric_coeff", C.c_double), ("update_nlist", C.c_int32),
-                ("step_offset", C.c_int32), ("total_steps", C.c_int32), ("seed", C.c_uint64)]
+                ("step_offset", C.c_int32), ("total_steps", C.c_int32), ("seed", C.c_uint64),
+                ("noise_stream", C.c_uint32)]
 
 
 def nb_params(**kw) -> NBParams:
@@ -142,6 +146,17 @@ def load():
     L.pdgpu_md.argtypes = [vp, C.POINTER(MDParams), C.c_int32, dp, dp]
     L.pdgpu_set_timing.argtypes = [vp, C.c_int32]
     L.pdgpu_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
+    L.pdgpu_batch_create.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(vp)]
+    L.pdgpu_batch_destroy.argtypes = [vp]
+    L.pdgpu_batch_last_error.argtypes = [vp]
+    L.pdgpu_batch_last_error.restype = C.c_char_p
+    L.pdgpu_batch_set_temperatures.argtypes = [vp, dp]
+    for f in ("pdgpu_batch_get_positions", "pdgpu_batch_set_positions", "pdgpu_batch_get_velocities",
+              "pdgpu_batch_set_velocities"):
+        getattr(L, f).argtypes = [vp, C.c_int32, dp, C.c_size_t]
+    L.pdgpu_batch_compute.argtypes = [vp, C.c_uint32, dp, C.POINTER(Energies)]
+    L.pdgpu_batch_md.argtypes = [vp, C.POINTER(MDParams), C.c_int32, dp, dp]
+    L.pdgpu_batch_remd.argtypes = [vp, C.POINTER(MDParams), C.c_int32, C.c_uint64, ip, ip, dp]
     _lib = L
     return L
 
@@ -325,6 +340,88 @@ class Engine:
                                   ep.ctypes.data_as(C.POINTER(C.c_double)),
                                   ek.ctypes.data_as(C.POINTER(C.c_double))))
         return ep[:n_steps], ek[:n_steps]
+
+
+class Batch:
+    """Independent systems advanced together (the pdgpu_batch handle)."""
+
+    def __init__(self, engines):
+        self.L = load()
+        self.engines = list(engines)
+        self.counts = [e.n for e in self.engines]
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = self.L.pdgpu_batch_create(len(self.engines), arr, C.byref(h))
+        if rc != 0:
+            msg = "; ".join(self.L.pdgpu_last_error(e.h).decode() for e in self.engines)
+            raise PdGpuError(rc, f"pdgpu_batch_create failed: {msg}")
+        self.h = h
+        self.nsys = len(self.engines)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pdgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PdGpuError(rc, self.L.pdgpu_batch_last_error(self.h).decode())
+
+    def set_temperatures(self, temps):
+        t, tp = _d(temps)
+        self._chk(self.L.pdgpu_batch_set_temperatures(self.h, tp))
+
+    def _state(self, fn, s, arr=None):
+        if arr is None:
+            out = np.zeros((self.counts[s], 3))
+            self._chk(fn(self.h, int(s), out.ctypes.data_as(C.POINTER(C.c_double)), 24))
+            return out
+        a, ap = _d(np.asarray(arr).reshape(-1, 3))
+        self._chk(fn(self.h, int(s), ap, 24))
+
+    def get_positions(self, s):
+        return self._state(self.L.pdgpu_batch_get_positions, s)
+
+    def set_positions(self, s, pos):
+        self._state(self.L.pdgpu_batch_set_positions, s, pos)
+
+    def get_velocities(self, s):
+        return self._state(self.L.pdgpu_batch_get_velocities, s)
+
+    def set_velocities(self, s, vel):
+        self._state(self.L.pdgpu_batch_set_velocities, s, vel)
+
+    def compute(self, components: int, forces: bool = True):
+        f = np.zeros((sum(self.counts), 3))
+        e = (Energies * self.nsys)()
+        fp = f.ctypes.data_as(C.POINTER(C.c_double)) if forces else None
+        self._chk(self.L.pdgpu_batch_compute(self.h, int(components), fp, e))
+        parts = np.split(f, np.cumsum(self.counts)[:-1]) if forces else None
+        return parts, [x.as_dict() for x in e]
+
+    def md(self, params: MDParams, n_steps: int):
+        ep = np.zeros((max(n_steps, 1), self.nsys))
+        ek = np.zeros((max(n_steps, 1), self.nsys))
+        self._chk(self.L.pdgpu_batch_md(self.h, C.byref(params), int(n_steps),
+                                        ep.ctypes.data_as(C.POINTER(C.c_double)),
+                                        ek.ctypes.data_as(C.POINTER(C.c_double))))
+        return ep[:n_steps], ek[:n_steps]
+
+    def remd(self, params: MDParams, n_rounds: int, seed: int):
+        acc = np.zeros(self.nsys, np.int32)
+        att = np.zeros(self.nsys, np.int32)
+        fe = np.zeros(self.nsys)
+        self._chk(self.L.pdgpu_batch_remd(self.h, C.byref(params), int(n_rounds), int(seed),
+                                          acc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          att.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          fe.ctypes.data_as(C.POINTER(C.c_double))))
+        return acc, att, fe
 
 
 def engine_for(sys, nb: Optional[NBParams] = None, padding: float = 0.5, calc_shadow: bool = False,
