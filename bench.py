@@ -91,6 +91,135 @@ def cpu_baseline(sysw, nb_kw, padding):
     }
 
 
+GB_KW = dict(Cutoff=999.0, InnerCutoff=999.0, VdwCutoff=999.0, VdwInnerCutoff=999.0, GbsaStillCutoff=999.0,
+             DielectricSolute=1.0, DielectricSolvent=78.5, Vdw14Scaling=0.5, Elec14Scaling=0.83333)
+# algorithmic FLOPs per unordered pair and step of the three GB passes: pair pass 86,
+# Born radii 2 x 16, chain rule 2 x 28 (174 in total)
+GB_FLOP_PAIR, GB_FLOP_RADII, GB_FLOP_CHAIN = 86, 32, 56
+
+
+def _build_chain(args):
+    from pd_amd import systems as S
+    n_res, seed = args
+    return S.peptide_chain(n_res, seed=seed)
+
+
+def build_chains(specs, workers):
+    if workers <= 1 or len(specs) < 4:
+        return [_build_chain(s) for s in specs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_build_chain, specs, chunksize=max(1, len(specs) // (4 * workers)))
+
+
+def gb_engine(G, s, seed):
+    from pd_amd import units as U
+    e = G.engine_for(s, None, bonded=True)
+    e.setup_gb(G.gb_params(**GB_KW), s.gbrad, s.cov)
+    rng = np.random.default_rng(seed)
+    e.set_velocities(rng.normal(size=(s.n, 3)) * np.sqrt(U.KB * 300.0 / s.mass)[:, None])
+    return e
+
+
+def gb_benchmarks(args, log):
+    """The GB/SA workloads: one 300-residue chain, 256 uneven chains batched vs the
+    single-system engine looped over them, and 64 REMD replicas."""
+    import torch
+    from pd_amd import engine as G
+    comps = G.PDGPU_GB | G.PDGPU_BONDED
+    lang = dict(integrator=G.PDGPU_LANGEVIN, components=comps, timestep=2e-15, temperature=300.0,
+                fric_coeff=1e12, update_nlist=0, step_offset=0, total_steps=1 << 30, seed=3, noise_stream=0)
+    relax = dict(lang, fric_coeff=5e13, temperature=100.0, timestep=1e-15)
+    workers = min(16, os.cpu_count() or 1)
+    out = {}
+
+    # --- single 300-residue chain (all pairs) ---------------------------------
+    t0 = time.perf_counter()
+    s = build_chains([(300, 7)], 1)[0]
+    e = gb_engine(G, s, 1)
+    e.md(G.MDParams(**relax), 200)
+    e.md(G.MDParams(**lang), 10)
+    pairs = s.n * (s.n - 1) // 2
+    e.set_timing(True)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    e.md(G.MDParams(**dict(lang, step_offset=10)), args.gb_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - ts
+    kt = {k: e.kernel_time(k) for k in ("gb_radii", "gb_pair", "gb_chain")}
+    e.set_timing(False)
+    pair_us = kt["gb_pair"][0] / max(kt["gb_pair"][1], 1) * 1e3
+    achieved = pairs * GB_FLOP_PAIR / (pair_us * 1e-6) / 1e12
+    out["single_chain"] = {
+        "workload": "gbsa_random_chain_300res", "n_atoms": s.n, "steps": args.gb_steps,
+        "steps_per_s": round(args.gb_steps / wall, 2), "ms_per_step": round(wall * 1e3 / args.gb_steps, 4),
+        "ns_per_day": round(args.gb_steps * 2e-6 * 86400 / wall, 3), "pairs": pairs,
+        "kernel_us": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt.items()},
+        "roofline_gb_pair": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                             "algorithmic_flop_per_launch": pairs * GB_FLOP_PAIR},
+    }
+    log(f"single chain: {out['single_chain']['steps_per_s']} steps/s ({time.perf_counter() - t0:.1f} s)")
+    del e
+
+    # --- 256 uneven chains: batched vs looping the single-system engine --------
+    rng = np.random.default_rng(2024)
+    lengths = rng.integers(20, 121, args.gb_systems)
+    t0 = time.perf_counter()
+    chains = build_chains([(int(n), 1000 + k) for k, n in enumerate(lengths)], workers)
+    log(f"built {len(chains)} chains ({sum(c.n for c in chains)} atoms) in {time.perf_counter() - t0:.1f} s")
+    engines = [gb_engine(G, c, 100 + k) for k, c in enumerate(chains)]
+    batch = G.Batch(engines)
+    batch.md(G.MDParams(**relax), 200)
+    batch.md(G.MDParams(**lang), 5)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    batch.md(G.MDParams(**dict(lang, step_offset=5)), args.gb_batch_steps)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - ts
+    agg_b = len(chains) * args.gb_batch_steps / wall_b
+    for en in engines:  # warm each single-system engine
+        en.md(G.MDParams(**lang), 2)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for en in engines:
+        en.md(G.MDParams(**dict(lang, step_offset=2)), args.gb_plain_steps)
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - ts
+    agg_p = len(chains) * args.gb_plain_steps / wall_p
+    pair_total = sum(c.n * (c.n - 1) // 2 for c in chains)
+    out["batched_chains"] = {
+        "workload": f"gbsa_{len(chains)}_chains_20-120res", "n_systems": len(chains),
+        "n_atoms": int(sum(c.n for c in chains)), "pairs_per_step": int(pair_total),
+        "batched_steps_per_s": round(agg_b, 1), "plain_loop_steps_per_s": round(agg_p, 1),
+        "speedup_vs_plain": round(agg_b / agg_p, 2), "batched_ms_per_step": round(wall_b * 1e3 / args.gb_batch_steps, 3),
+        "gflop_per_step": round(pair_total * (GB_FLOP_PAIR + GB_FLOP_RADII + GB_FLOP_CHAIN) / 1e9, 3),
+    }
+    log(f"batched: {agg_b:.1f} vs plain {agg_p:.1f} aggregate steps/s")
+    del batch, engines, chains
+
+    # --- REMD: 64 replicas of a 60-residue chain, 300-500 K ------------------
+    base = build_chains([(60, 77)], 1)[0]
+    temps = 300.0 * (500.0 / 300.0) ** (np.arange(args.remd_replicas) / (args.remd_replicas - 1))
+    reps = [gb_engine(G, base, 500 + k) for k in range(args.remd_replicas)]
+    rb = G.Batch(reps)
+    rb.set_temperatures(temps)
+    rb.md(G.MDParams(**relax), 100)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    acc, att, fe = rb.remd(G.MDParams(**dict(lang, total_steps=args.remd_interval)), args.remd_rounds, 11)
+    torch.cuda.synchronize()
+    wall_r = time.perf_counter() - ts
+    out["remd"] = {
+        "workload": f"remd_{args.remd_replicas}x_{base.n}atoms_300-500K", "rounds": args.remd_rounds,
+        "exchange_interval": args.remd_interval,
+        "aggregate_steps_per_s": round(args.remd_replicas * args.remd_rounds * args.remd_interval / wall_r, 1),
+        "acceptance": [round(a / t, 3) if t else None for a, t in zip(acc[:-1].tolist(), att[:-1].tolist())],
+    }
+    log(f"remd: {out['remd']['aggregate_steps_per_s']} aggregate steps/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +228,14 @@ def main():
     ap.add_argument("--relax", type=int, default=300, help="untimed Langevin relaxation steps")
     ap.add_argument("--waters", type=int, default=32768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gb", action="store_true", help="skip the GB/SA workloads")
+    ap.add_argument("--gb-steps", type=int, default=200)
+    ap.add_argument("--gb-systems", type=int, default=256)
+    ap.add_argument("--gb-batch-steps", type=int, default=100)
+    ap.add_argument("--gb-plain-steps", type=int, default=20)
+    ap.add_argument("--remd-replicas", type=int, default=64)
+    ap.add_argument("--remd-rounds", type=int, default=2)
+    ap.add_argument("--remd-interval", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -202,6 +339,9 @@ def main():
             "algorithmic_bytes": int(w.n * 24 + n_tot * 4),
         },
     }
+    if not args.no_gb:
+        del eng
+        out["gbsa"] = gb_benchmarks(args, log)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle, 1 thread)...")
         cb = cpu_baseline(w, nb_kw, padding)

@@ -348,15 +348,20 @@ def _relieve_clashes(x, bonds, nbrs, is_h, iters=300):
                 if i < j:
                     c13.add((i, j))
     cons = np.array(sorted(c13), dtype=np.int64).reshape(-1, 2)
+    ck = np.sort(cons[:, 0] * n + cons[:, 1])
     d0 = np.linalg.norm(x[cons[:, 0]] - x[cons[:, 1]], axis=1)
+
+    def scatter(idx_a, idx_b, s):
+        for ax in range(3):
+            x[:, ax] += np.bincount(idx_b, s[:, ax], minlength=n) - np.bincount(idx_a, s[:, ax], minlength=n)
+
     for _ in range(iters):
-        tree = cKDTree(x)
-        pr = tree.query_pairs(2.7, output_type="ndarray")
+        pr = cKDTree(x).query_pairs(2.7, output_type="ndarray")
+        moved = False
         if len(pr):
             key = pr[:, 0] * n + pr[:, 1]
-            ck = cons[:, 0] * n + cons[:, 1]
-            pr = pr[~np.isin(key, ck)]
-        moved = False
+            pos = np.clip(np.searchsorted(ck, key), 0, len(ck) - 1)
+            pr = pr[ck[pos] != key]
         if len(pr):
             lim = np.where(is_h[pr[:, 0]] | is_h[pr[:, 1]], 2.0, 2.7)
             v = x[pr[:, 1]] - x[pr[:, 0]]
@@ -365,15 +370,11 @@ def _relieve_clashes(x, bonds, nbrs, is_h, iters=300):
             if bad.any():
                 moved = True
                 v, d, lim, p = v[bad], d[bad], lim[bad], pr[bad]
-                s = (0.5 * (lim - d) / np.maximum(d, 1e-6))[:, None] * v
-                np.add.at(x, p[:, 0], -s)
-                np.add.at(x, p[:, 1], s)
+                scatter(p[:, 0], p[:, 1], (0.5 * (lim - d) / np.maximum(d, 1e-6))[:, None] * v)
         for _k in range(3):
             v = x[cons[:, 1]] - x[cons[:, 0]]
             d = np.linalg.norm(v, axis=1)
-            s = (0.5 * (d - d0) / np.maximum(d, 1e-9))[:, None] * v
-            np.add.at(x, cons[:, 0], s * 0.5)
-            np.add.at(x, cons[:, 1], -s * 0.5)
+            scatter(cons[:, 1], cons[:, 0], (0.25 * (d - d0) / np.maximum(d, 1e-9))[:, None] * v)
         if not moved:
             break
     return x
