@@ -220,6 +220,20 @@ def gb_benchmarks(args, log):
     return out
 
 
+def copy_probe(mib):
+    """A plain copy of known size: the reference point that converts the HBM byte
+    counters of a profiler run (FETCH_SIZE / WRITE_SIZE) into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(20):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 20}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,11 +250,16 @@ def main():
     ap.add_argument("--remd-replicas", type=int, default=64)
     ap.add_argument("--remd-rounds", type=int, default=2)
     ap.add_argument("--remd-interval", type=int, default=500)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 20 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import torch
     from pd_amd import engine as G

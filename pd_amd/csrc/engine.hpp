@@ -6,10 +6,11 @@
 //       kcal/mol/A), molecules (first,last), bond-order band/sparse tables.
 //   cluster order (rebuilt with the list): slots of 8 atoms per cluster, atoms
 //       of a cluster taken from one grid column sorted by z.  slot_orig[slot]
-//       maps back to the PD index (-1 for padding); sx,sy,sz (f64) hold the
-//       positions gathered each step; sq,sr,se (f32) the scaled charge, LJ
-//       radius and sqrt(eps) per slot; nbx,nby,nbz (int64) the nonbonded force
-//       accumulators in slot order.
+//       maps back to the PD index (-1 for padding); spos (f64 x,y,z + pad, 32 B)
+//       holds the positions gathered each step, sprm (f32 scaled charge, LJ
+//       radius, sqrt(eps), 16 B) the pair parameters; nbx,nby,nbz (int64) the nonbonded force
+//       accumulators in slot order.  Work groups of up to 4 clusters of one
+//       column share a candidate j-cluster table (wg_cand).
 //   list: for every i-cluster a contiguous run of 8-byte entries
 //       {j slot | image<<24 ; listed-mask(8) | bond-order codes(16)} (see
 //       k_list.hip), located by cl_entry_start / cl_entry_count.
@@ -33,6 +34,13 @@ namespace pdgpu {
 constexpr double kFixScale = 4294967296.0;
 constexpr float kFixScaleF = 4294967296.0f;
 constexpr int kClusterSize = 8;
+// Nonbonded work groups: up to kGroup consecutive i-clusters of one grid column,
+// one wavefront each.  Their list shares one table of up to kGroupRanks
+// candidate j-clusters, which the nonbonded kernel mirrors in LDS to sum the j
+// forces before touching global memory (rank kGroupRanks = no LDS slot).
+constexpr int kGroup = 4;
+constexpr int kGroupRanks = 255;
+constexpr int kGroupStride = 256;
 constexpr int kWave = 64;
 
 // PD constants (forcefield.cpp:35-55), FP64.
@@ -99,6 +107,12 @@ struct DBuf {
   void download(T *h, size_t count, hipStream_t s) const {
     if (count) PDGPU_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
   }
+};
+
+// Position of one cluster slot: one 32-byte record, so a lane fetches it with
+// two 16-byte loads.
+struct alignas(32) SlotPos {
+  double x, y, z, pad;
 };
 
 // Parameters the nonbonded kernel needs, in the kernel's units (kcal/mol, A).
@@ -232,8 +246,11 @@ struct Engine {
   DBuf<int> n_clusters_d;                  // device scalar
   DBuf<int> cl_col;                        // column of each cluster
   DBuf<double> cl_lo, cl_hi;               // cluster bounding boxes (wrapped frame), 3 per cluster
-  DBuf<double> sx, sy, sz;                 // gathered positions, slot order
-  DBuf<float> sq, sr, se;                  // per-slot parameters
+  DBuf<SlotPos> spos;                      // gathered FP64 positions, slot order
+  DBuf<float4> sprm;                       // per slot: scaled charge, LJ radius, sqrt(eps), 0
+  int max_groups = 0;                      // allocation bound of work groups
+  DBuf<int> col_grp_start, grp_first, grp_count, n_groups_d;
+  DBuf<int> wg_ncand, wg_cand;             // per group: candidate j-cluster table (kGroupStride each)
   DBuf<long long> nbx, nby, nbz;           // nonbonded force accumulators, slot order
   DBuf<uint2> entries;
   DBuf<int> cl_entry_start, cl_entry_count;
@@ -345,12 +362,17 @@ void reduce4(hipStream_t s, const double *part, int count, double *out);
 
 // ---- device helpers ------------------------------------------------------------
 
-__device__ __forceinline__ long long fix_from_float(float v) {
-  return static_cast<long long>(v * kFixScaleF);
-}
+// floor(v * 2^32) as a 64-bit integer, exactly: d = v * 2^32 is exact, hi is its
+// upper 32 bits (floor), lo = d - hi * 2^32 lies in [0, 2^32) and is exact, so
+// two 32-bit conversions build the result (|v| < 2^31 kcal/mol/A).
 __device__ __forceinline__ long long fix_from_double(double v) {
-  return static_cast<long long>(v * kFixScale);
+  const double d = v * kFixScale;
+  const double hi = floor(d * (1.0 / kFixScale));
+  const double lo = fma(hi, -kFixScale, d);
+  const unsigned long long h = (unsigned long long)(unsigned)(int)hi;
+  return (long long)((h << 32) | (unsigned long long)(unsigned)lo);
 }
+__device__ __forceinline__ long long fix_from_float(float v) { return fix_from_double((double)v); }
 __device__ __forceinline__ void fix_add(long long *p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }

@@ -123,32 +123,39 @@ __global__ void k_bin(int n, GridArgs g, const double *__restrict__ x, const dou
 // Single-workgroup exclusive scans over the columns.
 __global__ void k_scan_columns(int ncol, const int *__restrict__ col_count, int *__restrict__ col_atom_start,
                                int *__restrict__ col_cl_start, int *__restrict__ col_ncl,
-                               int *__restrict__ col_fill, int *__restrict__ n_clusters) {
-  __shared__ int sa[1024], sc[1024];
-  __shared__ int carry_a, carry_c;
+                               int *__restrict__ col_fill, int *__restrict__ col_grp_start,
+                               int *__restrict__ n_clusters, int *__restrict__ n_groups) {
+  __shared__ int sa[1024], sc[1024], sg[1024];
+  __shared__ int carry_a, carry_c, carry_g;
   if (threadIdx.x == 0) {
     carry_a = 0;
     carry_c = 0;
+    carry_g = 0;
   }
   __syncthreads();
   for (int base = 0; base < ncol; base += 1024) {
     int c = base + threadIdx.x;
     int cnt = c < ncol ? col_count[c] : 0;
     int ncl = (cnt + kClusterSize - 1) / kClusterSize;
+    int ngr = (ncl + kGroup - 1) / kGroup;
     sa[threadIdx.x] = cnt;
     sc[threadIdx.x] = ncl;
+    sg[threadIdx.x] = ngr;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
       int va = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0;
       int vc = threadIdx.x >= (unsigned)off ? sc[threadIdx.x - off] : 0;
+      int vg = threadIdx.x >= (unsigned)off ? sg[threadIdx.x - off] : 0;
       __syncthreads();
       sa[threadIdx.x] += va;
       sc[threadIdx.x] += vc;
+      sg[threadIdx.x] += vg;
       __syncthreads();
     }
     if (c < ncol) {
       col_atom_start[c] = carry_a + sa[threadIdx.x] - cnt;
       col_cl_start[c] = carry_c + sc[threadIdx.x] - ncl;
+      col_grp_start[c] = carry_g + sg[threadIdx.x] - ngr;
       col_ncl[c] = ncl;
       col_fill[c] = 0;
     }
@@ -156,10 +163,27 @@ __global__ void k_scan_columns(int ncol, const int *__restrict__ col_count, int 
     if (threadIdx.x == 1023) {
       carry_a += sa[1023];
       carry_c += sc[1023];
+      carry_g += sg[1023];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *n_clusters = carry_c;
+  if (threadIdx.x == 0) {
+    *n_clusters = carry_c;
+    *n_groups = carry_g;
+  }
+}
+
+// Work groups: the clusters of each column cut into runs of kGroup.
+__global__ void k_fill_groups(int ncol, const int *__restrict__ col_cl_start, const int *__restrict__ col_ncl,
+                              const int *__restrict__ col_grp_start, int *__restrict__ grp_first,
+                              int *__restrict__ grp_count) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const int ncl = col_ncl[c], g0 = col_grp_start[c], c0 = col_cl_start[c];
+  for (int k = 0; k * kGroup < ncl; k++) {
+    grp_first[g0 + k] = c0 + k * kGroup;
+    grp_count[g0 + k] = min(kGroup, ncl - k * kGroup);
+  }
 }
 
 __global__ void k_fill_columns(int n, const int *__restrict__ atom_col, const int *__restrict__ col_atom_start,
@@ -252,85 +276,112 @@ __device__ __forceinline__ unsigned spread8(unsigned m) {
   return m;
 }
 
-__global__ __launch_bounds__(64) void k_pair_search(
-    SearchArgs A, const int *__restrict__ n_clusters, const int *__restrict__ cl_col,
-    const int *__restrict__ col_cl_start, const int *__restrict__ col_ncl,
-    const double *__restrict__ cl_lo, const double *__restrict__ cl_hi, const int *__restrict__ slot_orig,
-    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-    const double *__restrict__ wx, const double *__restrict__ wy, const double *__restrict__ wz,
-    const int8_t *__restrict__ band, const int *__restrict__ sp_off, const int *__restrict__ sp_part,
-    const int *__restrict__ sp_ord, uint2 *__restrict__ entries, int *__restrict__ cl_entry_count,
+__global__ __launch_bounds__(64 * kGroup) void k_pair_search(
+    SearchArgs A, const int *__restrict__ n_groups, const int *__restrict__ grp_first,
+    const int *__restrict__ grp_count, const int *__restrict__ cl_col, const int *__restrict__ col_cl_start,
+    const int *__restrict__ col_ncl, const double *__restrict__ cl_lo, const double *__restrict__ cl_hi,
+    const int *__restrict__ slot_orig, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ wx, const double *__restrict__ wy,
+    const double *__restrict__ wz, const int8_t *__restrict__ band, const int *__restrict__ sp_off,
+    const int *__restrict__ sp_part, const int *__restrict__ sp_ord, uint2 *__restrict__ entries,
+    int *__restrict__ cl_entry_count, int *__restrict__ wg_ncand, int *__restrict__ wg_cand,
     unsigned long long *__restrict__ counter) {
 #pragma clang fp contract(off)
   __shared__ int cand_cl[kMaxCand];
   __shared__ int cand_sh[kMaxCand];
-  const int ci = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (ci >= *n_clusters) return;
-
-  const double ilo0 = cl_lo[3 * ci], ilo1 = cl_lo[3 * ci + 1], ilo2 = cl_lo[3 * ci + 2];
-  const double ihi0 = cl_hi[3 * ci], ihi1 = cl_hi[3 * ci + 1], ihi2 = cl_hi[3 * ci + 2];
-  const int col = cl_col[ci];
+  __shared__ int s_ncand;
+  const int g = blockIdx.x;
+  if (g >= *n_groups) return;  // the whole workgroup leaves together
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = grp_first[g], ng = grp_count[g];
+  double glo[3], ghi[3];  // bounding box of the group's clusters
+  for (int d = 0; d < 3; d++) {
+    glo[d] = cl_lo[3 * c0 + d];
+    ghi[d] = cl_hi[3 * c0 + d];
+  }
+  for (int k = 1; k < ng; k++)
+    for (int d = 0; d < 3; d++) {
+      glo[d] = fmin(glo[d], cl_lo[3 * (c0 + k) + d]);
+      ghi[d] = fmax(ghi[d], cl_hi[3 * (c0 + k) + d]);
+    }
+  const int col = cl_col[c0];
   const int cx = col % A.ncx, cy = col / A.ncx;
 
-  // ---- phase 1: candidate j-clusters, deterministic order -------------------
-  int ncand = 0;
-  bool cand_overflow = false;
-  const int kz_lo = A.periodic ? -1 : 0, kz_hi = A.periodic ? 1 : 0;
-  for (int dy = -A.Dy; dy <= A.Dy; dy++) {
-    int uy = cy + dy, ky = 0;
-    if (A.periodic) {
-      for (int it = 0; it < 4 && uy < 0; it++) { uy += A.ncy; ky -= 1; }
-      for (int it = 0; it < 4 && uy >= A.ncy; it++) { uy -= A.ncy; ky += 1; }
-      if (ky < -1 || ky > 1 || uy < 0 || uy >= A.ncy) continue;
-    } else if (uy < 0 || uy >= A.ncy) {
-      continue;
+  // ---- phase 1 (first wave): candidate j-clusters in a fixed order -----------
+  // The group's own clusters come first (ranks 0..ng-1), then every cluster of
+  // the column stencil, image by image, whose box comes within reach of the
+  // group's box.  Only j-clusters >= the group's first cluster are candidates
+  // (each unordered cluster pair is searched from its lower cluster).
+  if (wave == 0) {
+    if (lane < ng) {
+      cand_cl[lane] = c0 + lane;
+      cand_sh[lane] = 13;
     }
-    for (int dx = -A.Dx; dx <= A.Dx; dx++) {
-      int ux = cx + dx, kx = 0;
+    int ncand = ng;
+    const int kz_lo = A.periodic ? -1 : 0, kz_hi = A.periodic ? 1 : 0;
+    for (int dy = -A.Dy; dy <= A.Dy; dy++) {
+      int uy = cy + dy, ky = 0;
       if (A.periodic) {
-        for (int it = 0; it < 4 && ux < 0; it++) { ux += A.ncx; kx -= 1; }
-        for (int it = 0; it < 4 && ux >= A.ncx; it++) { ux -= A.ncx; kx += 1; }
-        if (kx < -1 || kx > 1 || ux < 0 || ux >= A.ncx) continue;
-      } else if (ux < 0 || ux >= A.ncx) {
+        for (int it = 0; it < 4 && uy < 0; it++) { uy += A.ncy; ky -= 1; }
+        for (int it = 0; it < 4 && uy >= A.ncy; it++) { uy -= A.ncy; ky += 1; }
+        if (ky < -1 || ky > 1 || uy < 0 || uy >= A.ncy) continue;
+      } else if (uy < 0 || uy >= A.ncy) {
         continue;
       }
-      const int jc = uy * A.ncx + ux;
-      const int cs = col_cl_start[jc], nc = col_ncl[jc];
-      const double shx = kx * A.L[0], shy = ky * A.L[1];
-      for (int kz = kz_lo; kz <= kz_hi; kz++) {
-        const double shz = kz * A.L[2];
-        for (int base = 0; base < nc; base += 64) {
-          int t = base + lane;
-          int cj = cs + t;
-          bool ok = (t < nc) && (cj >= ci);
-          if (ok) {
-            double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ihi0, ilo0 - (cl_hi[3 * cj] + shx)));
-            double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ihi1, ilo1 - (cl_hi[3 * cj + 1] + shy)));
-            double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ihi2, ilo2 - (cl_hi[3 * cj + 2] + shz)));
-            ok = (g0 * g0 + g1 * g1 + g2 * g2) <= A.Rbox2;
-          }
-          unsigned long long b = __ballot(ok);
-          int pre = __popcll(b & ((1ull << lane) - 1ull));
-          if (ok) {
-            int pos = ncand + pre;
-            if (pos < kMaxCand) {
-              cand_cl[pos] = cj;
-              cand_sh[pos] = (kx + 1) * 9 + (ky + 1) * 3 + (kz + 1);
+      for (int dx = -A.Dx; dx <= A.Dx; dx++) {
+        int ux = cx + dx, kx = 0;
+        if (A.periodic) {
+          for (int it = 0; it < 4 && ux < 0; it++) { ux += A.ncx; kx -= 1; }
+          for (int it = 0; it < 4 && ux >= A.ncx; it++) { ux -= A.ncx; kx += 1; }
+          if (kx < -1 || kx > 1 || ux < 0 || ux >= A.ncx) continue;
+        } else if (ux < 0 || ux >= A.ncx) {
+          continue;
+        }
+        const int jcol = uy * A.ncx + ux;
+        const int cs = col_cl_start[jcol], nc = col_ncl[jcol];
+        const double shx = kx * A.L[0], shy = ky * A.L[1];
+        for (int kz = kz_lo; kz <= kz_hi; kz++) {
+          const double shz = kz * A.L[2];
+          const int sh = (kx + 1) * 9 + (ky + 1) * 3 + (kz + 1);
+          for (int base = 0; base < nc; base += 64) {
+            const int t = base + lane;
+            const int cj = cs + t;
+            bool ok = (t < nc) && (cj >= c0) && !(sh == 13 && cj < c0 + ng);
+            if (ok) {
+              double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ghi[0], glo[0] - (cl_hi[3 * cj] + shx)));
+              double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ghi[1], glo[1] - (cl_hi[3 * cj + 1] + shy)));
+              double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ghi[2], glo[2] - (cl_hi[3 * cj + 2] + shz)));
+              ok = (g0 * g0 + g1 * g1 + g2 * g2) <= A.Rbox2;
             }
+            const unsigned long long b = __ballot(ok);
+            const int pre = __popcll(b & ((1ull << lane) - 1ull));
+            if (ok) {
+              const int pos = ncand + pre;
+              if (pos < kMaxCand) {
+                cand_cl[pos] = cj;
+                cand_sh[pos] = sh;
+              }
+            }
+            ncand += __popcll(b);
           }
-          ncand += __popcll(b);
         }
       }
     }
-  }
-  if (ncand > kMaxCand) {
-    cand_overflow = true;
-    ncand = kMaxCand;
+    if (lane == 0) s_ncand = ncand;
   }
   __syncthreads();
+  const bool cand_overflow = s_ncand > kMaxCand;
+  const int ncand = min(s_ncand, kMaxCand);
+  // the candidate table the nonbonded kernel mirrors in LDS (ranks < kGroupRanks)
+  const int nr = min(ncand, kGroupRanks);
+  for (int t = threadIdx.x; t < nr; t += blockDim.x) wg_cand[(size_t)g * kGroupStride + t] = cand_cl[t];
+  if (threadIdx.x == 0) wg_ncand[g] = nr;
+  if (wave >= ng) return;
 
-  // ---- phase 2: exact PD pair tests, 8 i atoms x 8 j atoms per candidate -----
+  // ---- phase 2 (one wave per i-cluster): exact PD pair tests, 8 x 8 per candidate
+  const int ci = c0 + wave;
+  const double ilo0 = cl_lo[3 * ci], ilo1 = cl_lo[3 * ci + 1], ilo2 = cl_lo[3 * ci + 2];
+  const double ihi0 = cl_hi[3 * ci], ihi1 = cl_hi[3 * ci + 1], ihi2 = cl_hi[3 * ci + 2];
   const int ii = lane & 7, jj = lane >> 3;
   const int sa = ci * kClusterSize + ii;
   const int oa = slot_orig[sa];
@@ -344,6 +395,15 @@ __global__ __launch_bounds__(64) void k_pair_search(
   for (int k = 0; k < ncand; k++) {
     const int cj = cand_cl[k];
     const int csh = cand_sh[k];
+    if (cj < ci) continue;  // searched from the lower cluster
+    {  // this cluster's own box against the candidate (uniform over the wave)
+      const double shx = (csh / 9 - 1) * A.L[0], shy = ((csh / 3) % 3 - 1) * A.L[1], shz = (csh % 3 - 1) * A.L[2];
+      double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ihi0, ilo0 - (cl_hi[3 * cj] + shx)));
+      double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ihi1, ilo1 - (cl_hi[3 * cj + 1] + shy)));
+      double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ihi2, ilo2 - (cl_hi[3 * cj + 2] + shz)));
+      if ((g0 * g0 + g1 * g1 + g2 * g2) > A.Rbox2) continue;
+    }
+    const unsigned rank = (unsigned)min(k, kGroupRanks);
     const int sb = cj * kClusterSize + jj;
     const int ob = slot_orig[sb];
     bool in = (oa >= 0) && (ob >= 0) && (cj != ci || sa < sb);
@@ -391,13 +451,14 @@ __global__ __launch_bounds__(64) void k_pair_search(
       unsigned long long hib = __ballot(match && (code & 2));
       unsigned gmask = 0;
 #pragma unroll
-      for (int g = 0; g < 8; g++) gmask |= (((mb >> (g * 8)) & 0xFFull) ? 1u : 0u) << g;
+      for (int q = 0; q < 8; q++) gmask |= (((mb >> (q * 8)) & 0xFFull) ? 1u : 0u) << q;
       if (ii == 0 && ((gmask >> jj) & 1u)) {
         int pos = cnt + __popc(gmask & ((1u << jj) - 1u));
         unsigned lm = (unsigned)((mb >> (jj * 8)) & 0xFFull);
         unsigned codes = spread8((unsigned)((lob >> (jj * 8)) & 0xFFull)) |
                          (spread8((unsigned)((hib >> (jj * 8)) & 0xFFull)) << 1);
-        if (pos < A.stride) E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | (codes << 8));
+        if (pos < A.stride)
+          E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | (codes << 8) | (rank << 24));
       }
       cnt += __popc(gmask);
       remaining &= ~mb;
@@ -463,6 +524,7 @@ void Engine::build_list() {
   ncy = g.ncy;
   ncol = ncx * ncy;
   max_clusters = n / kClusterSize + ncol + 1;
+  max_groups = ((max_clusters / kGroup + ncol + 1) + 7) / 8 * 8;
 
   wx.resize(n); wy.resize(n); wz.resize(n);
   atom_col.resize(n);
@@ -477,12 +539,20 @@ void Engine::build_list() {
   cl_hi.resize((size_t)max_clusters * 3);
   cl_entry_count.resize(max_clusters);
   entry_counter.resize(2);
+  col_grp_start.resize(ncol);
+  grp_first.resize(max_groups);
+  grp_count.resize(max_groups);
+  n_groups_d.resize(1);
+  wg_ncand.resize(max_groups);
+  wg_cand.resize((size_t)max_groups * kGroupStride);
 
   hipLaunchKernelGGL(k_fill_int, dim3((ncol + T - 1) / T), dim3(T), 0, stream, col_count.p, 0, ncol);
   hipLaunchKernelGGL(k_bin, dim3(nb_atoms), dim3(T), 0, stream, n, g, x.p, y.p, z.p, wx.p, wy.p, wz.p,
                      atom_col.p, col_count.p);
   hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, stream, ncol, col_count.p, col_atom_start.p,
-                     col_cl_start.p, col_ncl.p, col_fill.p, n_clusters_d.p);
+                     col_cl_start.p, col_ncl.p, col_fill.p, col_grp_start.p, n_clusters_d.p, n_groups_d.p);
+  hipLaunchKernelGGL(k_fill_groups, dim3((ncol + T - 1) / T), dim3(T), 0, stream, ncol, col_cl_start.p, col_ncl.p,
+                     col_grp_start.p, grp_first.p, grp_count.p);
   hipLaunchKernelGGL(k_fill_columns, dim3(nb_atoms), dim3(T), 0, stream, n, atom_col.p, col_atom_start.p,
                      col_fill.p, col_atoms.p);
   const int nslots = max_clusters * kClusterSize;
@@ -524,10 +594,10 @@ void Engine::build_list() {
     PDGPU_HIP(hipMemsetAsync(entry_counter.p, 0, 2 * sizeof(unsigned long long), stream));
     {
       ScopedTimer t(*this, "pair_search");
-      hipLaunchKernelGGL(k_pair_search, dim3(max_clusters), dim3(64), 0, stream, A, n_clusters_d.p, cl_col.p,
-                         col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p, slot_orig.p, x.p, y.p, z.p, wx.p, wy.p,
-                         wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p, entries.p, cl_entry_count.p,
-                         entry_counter.p);
+      hipLaunchKernelGGL(k_pair_search, dim3(max_groups), dim3(64 * kGroup), 0, stream, A, n_groups_d.p,
+                         grp_first.p, grp_count.p, cl_col.p, col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p,
+                         slot_orig.p, x.p, y.p, z.p, wx.p, wy.p, wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p,
+                         entries.p, cl_entry_count.p, wg_ncand.p, wg_cand.p, entry_counter.p);
     }
     PDGPU_HIP(hipGetLastError());
     unsigned long long hc[2];

@@ -1,21 +1,30 @@
 // k_nonbonded.hip -- FF_NonBonded::calcForces on the device.
 //
 // Restates FF_NonBonded_CalcForces_T_fast2 (nonbonded.cpp:559-910) over the
-// cluster list built by k_list.hip.  One wavefront owns one i-cluster (8 atoms,
-// held in scalar registers); its 64 lanes walk the cluster's list entries, one
-// j atom per lane, and loop over the 8 i atoms.  Per pair:
-//   * displacement and the cutoff test exactly as PD in FP64:
-//       d = (p_j - p_i) + basis[image],  r^2 = (x*x + y*y) + z*z,  keep r^2 <= Cutoff^2
-//     so the interacting pair set of every step is PD's, bit for bit;
+// cluster list built by k_list.hip.  A workgroup takes one work group of up to
+// four consecutive i-clusters of a grid column, one wavefront per i-cluster
+// (8 atoms, held in scalar registers).  The 64 lanes of a wave walk the
+// cluster's list entries, one j atom per lane, and loop over the 8 i atoms.
+// Per pair:
+//   * the displacement from FP64 positions relative to the i-cluster's first
+//     atom c: d = ((p_j - c) + basis[image]) - (p_i - c), rounded to FP32 once;
+//   * PD's cutoff test r^2 <= Cutoff^2 (nonbonded.cpp:718-722): decided from the
+//     FP32 r^2 whenever it lies more than kNearBand from Cutoff^2 (VdwCutoff^2),
+//     far beyond the FP32 error, and otherwise recomputed in FP64 exactly as PD
+//     does, d = (p_j - p_i) + basis[image], r^2 = (x*x + y*y) + z*z; so the
+//     interacting pair set of every step is PD's, bit for bit;
 //   * the pair energy/force in FP32 (kcal/mol, A) with PD's expressions:
 //     LJ 2eps(A/2 - B) with its energy switch (always on, nonbonded.cpp:748-761),
 //     Coulomb plain / energy-switched / force-switched (nonbonded.cpp:766-821);
 //   * 1-2/1-3 pairs (scale 0) drop out, 1-4 pairs use Vdw14/Elec14Scaling
 //     (nonbonded.cpp:650-651).
-// The j force of a lane is complete after the 8 i atoms and is added once, in
-// 64-bit fixed point; i forces stay in registers until the end of the cluster.
-// Fixed-point integer adds are order-independent, so the forces are bitwise
-// reproducible.  Energies go to per-wave FP64 partials reduced in a fixed order.
+// Forces are 64-bit fixed point.  Atomics to global memory run at a bounded
+// rate per lane operation on this GPU, so the j forces of the workgroup are
+// first summed in LDS: every list entry carries the rank of its j-cluster in
+// the work group's candidate table, and at the end the workgroup adds each
+// touched slot of the table to global memory once.  Integer adds are
+// order-independent, so the forces are bitwise reproducible.  Energies go to
+// per-wave FP64 partials reduced in a fixed order.
 #include "engine.hpp"
 
 #include <cmath>
@@ -26,34 +35,37 @@ namespace {
 
 enum { EL_NONE = 0, EL_PLAIN = 1, EL_ESWITCH = 3, EL_FSWITCH = 4 };
 
+// |r^2_fp32 - r^2_exact| is below 1e-5 r^2 (one FP32 rounding of d, three of the
+// sum); pairs within this band of a cutoff are re-tested in FP64.
+constexpr float kNearBand = 0.01f;
+
 __global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
                                const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, double *__restrict__ sx, double *__restrict__ sy,
-                               double *__restrict__ sz) {
+                               const double *__restrict__ z, SlotPos *__restrict__ spos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots || s >= *n_clusters * kClusterSize) return;
   int a = slot_orig[s];
+  SlotPos p;
   if (a >= 0) {
-    sx[s] = x[a];
-    sy[s] = y[a];
-    sz[s] = z[a];
+    p.x = x[a];
+    p.y = y[a];
+    p.z = z[a];
   } else {  // padding slot: far away, zero parameters, never listed
-    sx[s] = 1.0e6;
-    sy[s] = 1.0e6;
-    sz[s] = 1.0e6;
+    p.x = 1.0e6;
+    p.y = 1.0e6;
+    p.z = 1.0e6;
   }
+  p.pad = 0.0;
+  spos[s] = p;
 }
 
 __global__ void k_gather_params(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
                                 const float *__restrict__ q, const float *__restrict__ r,
-                                const float *__restrict__ e, float *__restrict__ sq, float *__restrict__ sr,
-                                float *__restrict__ se) {
+                                const float *__restrict__ e, float4 *__restrict__ sprm) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots || s >= *n_clusters * kClusterSize) return;
   int a = slot_orig[s];
-  sq[s] = a >= 0 ? q[a] : 0.0f;
-  sr[s] = a >= 0 ? r[a] : 0.0f;
-  se[s] = a >= 0 ? e[a] : 0.0f;
+  sprm[s] = a >= 0 ? make_float4(q[a], r[a], e[a], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ void k_zero_ll(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c,
@@ -73,171 +85,229 @@ __device__ __forceinline__ double image_shift(int img, int axis, double L) {
   return k == 1 ? 0.0 : (k == 2 ? L : -L);
 }
 
+__device__ __forceinline__ void lds_fix_add(long long *p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+
 template <int ELEC, bool VDW>
-__global__ __launch_bounds__(64) void k_nonbonded(
-    NBKernelParams P, const int *__restrict__ n_clusters, int grid_clusters, double Lx, double Ly,
-    double Lz, const double *__restrict__ sx, const double *__restrict__ sy, const double *__restrict__ sz,
-    const float *__restrict__ sq, const float *__restrict__ sr, const float *__restrict__ se,
-    const uint2 *__restrict__ entries, int stride, const int *__restrict__ cl_count,
-    long long *__restrict__ fx, long long *__restrict__ fy, long long *__restrict__ fz,
-    double *__restrict__ e_partial) {
+__global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_nonbonded(
+    NBKernelParams P, const int *__restrict__ n_groups, const int *__restrict__ grp_first,
+    const int *__restrict__ grp_count, const int *__restrict__ wg_ncand, const int *__restrict__ wg_cand,
+    double Lx, double Ly, double Lz, const SlotPos *__restrict__ spos, const float4 *__restrict__ sprm,
+    const uint2 *__restrict__ entries,
+    int stride, const int *__restrict__ cl_count, long long *__restrict__ fx, long long *__restrict__ fy,
+    long long *__restrict__ fz, double *__restrict__ e_partial) {
+  __shared__ long long jacc[3][kGroupRanks * kClusterSize];
+  __shared__ int cand[kGroupRanks];
+  __shared__ double shtab[27 * 3];
+  __shared__ double ioff[kGroup][kClusterSize][4];  // i atoms: FP64 offset from the cluster's first atom
+  __shared__ float4 iprm[kGroup][kClusterSize];     // i atoms: q, LJ radius, sqrt(eps)
   // XCD-aware order: consecutive workgroups are spread round-robin over the 8
-  // XCDs, so give each XCD a contiguous run of (spatially adjacent) clusters.
+  // XCDs, so give each XCD a contiguous run of (spatially adjacent) groups.
   const int nblk = gridDim.x;
   const int per = nblk / 8;
-  const int ci = (per * 8 == nblk) ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
-  const int lane = threadIdx.x;
-  const int ncl = *n_clusters;
-  if (ci >= ncl) {
-    if (lane < 4) e_partial[4 * ci + lane] = 0.0;
+  const int g = (per * 8 == nblk) ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wid = g * kGroup + wave;  // energy partial slot of this wave
+  if (g >= *n_groups) {  // the whole workgroup leaves together
+    if (lane < 4) e_partial[4 * wid + lane] = 0.0;
     return;
   }
-  double xi[8], yi[8], zi[8];
-  float qi[8], ri[8], ei[8];
-#pragma unroll
-  for (int ii = 0; ii < 8; ii++) {
-    const int s = ci * kClusterSize + ii;
-    xi[ii] = sx[s];
-    yi[ii] = sy[s];
-    zi[ii] = sz[s];
-    qi[ii] = sq[s];
-    ri[ii] = sr[s];
-    ei[ii] = se[s];
+  const int nr = wg_ncand[g];
+  for (int t = threadIdx.x; t < 3 * kGroupRanks * kClusterSize; t += blockDim.x) (&jacc[0][0])[t] = 0;
+  for (int t = threadIdx.x; t < nr; t += blockDim.x) cand[t] = wg_cand[(size_t)g * kGroupStride + t];
+  if (threadIdx.x < 81) {
+    const int axis = threadIdx.x % 3;
+    shtab[threadIdx.x] = image_shift(threadIdx.x / 3, axis, axis == 0 ? Lx : (axis == 1 ? Ly : Lz));
   }
-  float fix[8], fiy[8], fiz[8];
-#pragma unroll
-  for (int ii = 0; ii < 8; ii++) fix[ii] = fiy[ii] = fiz[ii] = 0.0f;
-  double ev = 0.0, ee = 0.0, vir = 0.0;  // per-lane FP64 energy accumulators
+  __syncthreads();
 
-  const int cnt = cl_count[ci];
-  const uint2 *E = entries + (size_t)ci * stride;
-  for (int k0 = 0; k0 < cnt; k0 += 64) {
-    const int k = k0 + lane;
-    const bool valid = k < cnt;
-    const uint2 ent = valid ? E[k] : make_uint2(0u, 0u);
-    const int j = (int)(ent.x & 0xFFFFFFu);
-    const int img = (int)(ent.x >> 24);
-    const unsigned lm = ent.y & 0xFFu;
-    const unsigned codes = ent.y >> 8;
-    const unsigned clo = codes & 0x5555u, chi = (codes >> 1) & 0x5555u;
-    const unsigned excl = clo ^ chi;  // codes 1 and 2: 1-2 / 1-3 pairs, scale 0
-    const unsigned is14 = clo & chi;  // code 3: 1-4 pair
-    const double xj = sx[j], yj = sy[j], zj = sz[j];
-    const double shx = image_shift(img, 0, Lx), shy = image_shift(img, 1, Ly), shz = image_shift(img, 2, Lz);
-    const float qj = sq[j], rj = sr[j], ej = se[j];
-    float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
-    float ev8 = 0.0f, ee8 = 0.0f, vir8 = 0.0f;  // the 8 pairs of this entry
+  double ev = 0.0, ee = 0.0, vir = 0.0;  // per-lane FP64 energy accumulators
+  if (wave < grp_count[g]) {
+    // uniform over the wave: the i-cluster data below lives in scalar registers
+    const int ci = __builtin_amdgcn_readfirstlane(grp_first[g] + wave);
+    const SlotPos oi = spos[ci * kClusterSize];  // the cluster's first atom (always a real atom)
+    if (lane < kClusterSize) {
+      const SlotPos p = spos[ci * kClusterSize + lane];
+      ioff[wave][lane][0] = p.x - oi.x;
+      ioff[wave][lane][1] = p.y - oi.y;
+      ioff[wave][lane][2] = p.z - oi.z;
+      iprm[wave][lane] = sprm[ci * kClusterSize + lane];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float fix[8], fiy[8], fiz[8];
 #pragma unroll
-    for (int ii = 0; ii < 8; ii++) {
-      double dx, dy, dz, r2;
-      {
+    for (int a = 0; a < 8; a++) fix[a] = fiy[a] = fiz[a] = 0.0f;
+    const float cut2f = (float)P.cut2, vcut2f = (float)P.vdwcut2;
+    const bool split_vdw = P.vdwcut2 != P.cut2;
+
+    const int cnt = cl_count[ci];
+    const uint2 *E = entries + (size_t)ci * stride;
+    uint2 nxt = lane < cnt ? E[lane] : make_uint2(0u, 0u);
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const bool valid = k0 + lane < cnt;
+      const uint2 ent = nxt;
+      nxt = (k0 + 64 + lane < cnt) ? E[k0 + 64 + lane] : make_uint2(0u, 0u);  // prefetch the next round
+      const int j = (int)(ent.x & 0xFFFFFFu);
+      const int img = (int)((ent.x >> 24) & 31u);
+      const unsigned lm = valid ? (ent.y & 0xFFu) : 0u;
+      const unsigned codes = (ent.y >> 8) & 0xFFFFu;
+      const unsigned rank = ent.y >> 24;
+      const unsigned clo = codes & 0x5555u, chi = (codes >> 1) & 0x5555u;
+      const unsigned excl = clo ^ chi;  // codes 1 and 2: 1-2 / 1-3 pairs, scale 0
+      const unsigned is14 = clo & chi;  // code 3: 1-4 pair
+      const SlotPos pj = spos[j];
+      const float4 qj = sprm[j];
+      const double shx = shtab[3 * img], shy = shtab[3 * img + 1], shz = shtab[3 * img + 2];
+      const double ax = (pj.x - oi.x) + shx, ay = (pj.y - oi.y) + shy, az = (pj.z - oi.z) + shz;
+      float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
+      float ev8 = 0.0f, ee8 = 0.0f, vir8 = 0.0f;  // the 8 pairs of this entry
+      // Re-read the i atoms from LDS in every round instead of holding 8 atoms in
+      // registers: the empty asm hides that the index is loop-invariant.
+      int ib = wave * kClusterSize;
+      asm volatile("" : "+v"(ib));
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        const double *io = &ioff[0][0][0] + 4 * (ib + a);
+        const float dx = (float)(ax - io[0]), dy = (float)(ay - io[1]), dz = (float)(az - io[2]);
+        const float r2s_a = dx * dx + dy * dy + dz * dz;
+        bool inc = r2s_a <= cut2f, inv = r2s_a < vcut2f;
+        const bool near = fabsf(r2s_a - cut2f) < kNearBand || (split_vdw && fabsf(r2s_a - vcut2f) < kNearBand);
+        if (near && ((lm >> a) & 1u)) {  // rare: PD's FP64 test, nonbonded.cpp:718-722
 #pragma clang fp contract(off)
-        dx = (xj - xi[ii]) + shx;
-        dy = (yj - yi[ii]) + shy;
-        dz = (zj - zi[ii]) + shz;
-        r2 = dx * dx + dy * dy + dz * dz;
-      }
-      const bool on = ((lm >> ii) & 1u) && !((excl >> (2 * ii)) & 1u) && (r2 <= P.cut2);
-      const bool s14 = (is14 >> (2 * ii)) & 1u;
-      const float r2f = on ? (float)r2 : 1.0f;
-      const float rinv = __builtin_amdgcn_rsqf(r2f);
-      const float r = r2f * rinv;
-      const float rinv2 = rinv * rinv;
-      float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
-      if (VDW) {
-        const float eps = (s14 ? P.vdw14 : 1.0f) * ei[ii] * ej;
-        const float Rij = ri[ii] + rj;
-        const float s2 = Rij * Rij * rinv2;
-        const float B = s2 * s2 * s2;
-        const float A = B * B;
-        float e0 = eps * (A - 2.0f * B);
-        float d0 = -12.0f * eps * rinv * (A - B);
-        const float t = fmaxf(r - P.vdwinner, 0.0f);
-        const float S0 = 1.0f - (P.invw_vdw * t) * (P.invw_vdw * t);
-        const float dS = -P.invw2_vdw4 * t * S0;
-        const float S = S0 * S0;
-        d0 = S * d0 + e0 * dS;
-        e0 = e0 * S;
-        const bool vin = r2 < P.vdwcut2;
-        Ev = vin ? e0 : 0.0f;
-        dEdr += vin ? d0 : 0.0f;
-      }
-      if (ELEC != EL_NONE) {
-        const float qq = (s14 ? P.elec14 : 1.0f) * qi[ii] * qj;
-        if (ELEC == EL_PLAIN) {
-          Ee = qq * rinv;
-          dEdr += -Ee * rinv;
-        } else if (ELEC == EL_ESWITCH) {
-          float e0 = qq * rinv;
-          float d0 = -e0 * rinv;
-          const float t = fmaxf(r - P.inner, 0.0f);
-          const float S0 = 1.0f - (P.invw_el * t) * (P.invw_el * t);
-          const float dS = -P.invw2_el4 * t * S0;
+          const SlotPos pi = spos[ci * kClusterSize + a];
+          const double ex = (pj.x - pi.x) + shx, ey = (pj.y - pi.y) + shy, ez = (pj.z - pi.z) + shz;
+          const double r2 = ex * ex + ey * ey + ez * ez;
+          inc = r2 <= P.cut2;
+          inv = r2 < P.vdwcut2;
+        }
+        const float4 pa = (&iprm[0][0])[ib + a];
+        const bool on = ((lm >> a) & 1u) && !((excl >> (2 * a)) & 1u) && inc;
+        const bool s14 = (is14 >> (2 * a)) & 1u;
+        const float r2f = on ? r2s_a : 1.0f;
+        const float rinv = __builtin_amdgcn_rsqf(r2f);
+        const float r = r2f * rinv;
+        const float rinv2 = rinv * rinv;
+        float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
+        if (VDW) {
+          const float eps = (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z;
+          const float Rij = pa.y + qj.y;
+          const float s2 = Rij * Rij * rinv2;
+          const float B = s2 * s2 * s2;
+          const float A = B * B;
+          float e0 = eps * (A - 2.0f * B);
+          float d0 = -12.0f * eps * rinv * (A - B);
+          const float t = fmaxf(r - P.vdwinner, 0.0f);
+          const float S0 = 1.0f - (P.invw_vdw * t) * (P.invw_vdw * t);
+          const float dS = -P.invw2_vdw4 * t * S0;
           const float S = S0 * S0;
           d0 = S * d0 + e0 * dS;
-          Ee = e0 * S;
-          dEdr += d0;
-        } else {  // force switch (Steinbach & Brooks), nonbonded.cpp:798-819
-          const bool outer = r > P.inner;
-          const float dc = P.rc2 - r2f;
-          const float d_out = -qq * P.sA * rinv2 * dc * dc * (P.rc2 - 3.0f * P.rin2 + 2.0f * r2f);
-          const float e_out = -qq * (P.sA * r * (P.sB * rinv2 + P.sC + P.sD * r2f + P.sE * r2f * r2f) - P.fcut);
-          const float d_in = -qq * rinv2;
-          const float e_in = qq * (rinv - P.eshift);
-          Ee = outer ? e_out : e_in;
-          dEdr += outer ? d_out : d_in;
+          e0 = e0 * S;
+          Ev = inv ? e0 : 0.0f;
+          dEdr += inv ? d0 : 0.0f;
+        }
+        if (ELEC != EL_NONE) {
+          const float qq = (s14 ? P.elec14 : 1.0f) * pa.x * qj.x;
+          if (ELEC == EL_PLAIN) {
+            Ee = qq * rinv;
+            dEdr += -Ee * rinv;
+          } else if (ELEC == EL_ESWITCH) {
+            float e0 = qq * rinv;
+            float d0 = -e0 * rinv;
+            const float t = fmaxf(r - P.inner, 0.0f);
+            const float S0 = 1.0f - (P.invw_el * t) * (P.invw_el * t);
+            const float dS = -P.invw2_el4 * t * S0;
+            const float S = S0 * S0;
+            d0 = S * d0 + e0 * dS;
+            Ee = e0 * S;
+            dEdr += d0;
+          } else {  // force switch (Steinbach & Brooks), nonbonded.cpp:798-819
+            const bool outer = r > P.inner;
+            const float dc = P.rc2 - r2f;
+            const float d_out = -qq * P.sA * rinv2 * dc * dc * (P.rc2 - 3.0f * P.rin2 + 2.0f * r2f);
+            const float e_out = -qq * (P.sA * r * (P.sB * rinv2 + P.sC + P.sD * r2f + P.sE * r2f * r2f) - P.fcut);
+            const float d_in = -qq * rinv2;
+            const float e_in = qq * (rinv - P.eshift);
+            Ee = outer ? e_out : e_in;
+            dEdr += outer ? d_out : d_in;
+          }
+        }
+        const float sc = on ? dEdr * rinv : 0.0f;
+        const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
+        fix[a] += gx;
+        fiy[a] += gy;
+        fiz[a] += gz;
+        fjx -= gx;
+        fjy -= gy;
+        fjz -= gz;
+        ev8 += on ? Ev : 0.0f;
+        ee8 += on ? Ee : 0.0f;
+        vir8 += on ? r * dEdr : 0.0f;
+      }
+      ev += (double)ev8;
+      ee += (double)ee8;
+      vir += (double)vir8;
+      if (lm) {
+        const long long vx = fix_from_float(fjx), vy = fix_from_float(fjy), vz = fix_from_float(fjz);
+        if (rank < (unsigned)kGroupRanks) {
+          const int l = (int)rank * kClusterSize + (j & (kClusterSize - 1));
+          lds_fix_add(&jacc[0][l], vx);
+          lds_fix_add(&jacc[1][l], vy);
+          lds_fix_add(&jacc[2][l], vz);
+        } else {
+          fix_add(&fx[j], vx);
+          fix_add(&fy[j], vy);
+          fix_add(&fz[j], vz);
         }
       }
-      const float sc = on ? dEdr * rinv : 0.0f;
-      const float gx = (float)dx * sc, gy = (float)dy * sc, gz = (float)dz * sc;
-      fix[ii] += gx;
-      fiy[ii] += gy;
-      fiz[ii] += gz;
-      fjx -= gx;
-      fjy -= gy;
-      fjz -= gz;
-      ev8 += on ? Ev : 0.0f;
-      ee8 += on ? Ee : 0.0f;
-      vir8 += on ? r * dEdr : 0.0f;
     }
-    ev += (double)ev8;
-    ee += (double)ee8;
-    vir += (double)vir8;
-    if (valid && lm) {
-      fix_add(&fx[j], fix_from_float(fjx));
-      fix_add(&fy[j], fix_from_float(fjy));
-      fix_add(&fz[j], fix_from_float(fjz));
-    }
-  }
+    // i forces: the cluster is rank `wave` of its own group's table
 #pragma unroll
-  for (int ii = 0; ii < 8; ii++) {
-    const float gx = wave_sum_f(fix[ii]);
-    const float gy = wave_sum_f(fiy[ii]);
-    const float gz = wave_sum_f(fiz[ii]);
-    if (lane == ii) {
-      const int s = ci * kClusterSize + ii;
-      fix_add(&fx[s], fix_from_float(gx));
-      fix_add(&fy[s], fix_from_float(gy));
-      fix_add(&fz[s], fix_from_float(gz));
+    for (int a = 0; a < 8; a++) {
+      const float gx = wave_sum_f(fix[a]);
+      const float gy = wave_sum_f(fiy[a]);
+      const float gz = wave_sum_f(fiz[a]);
+      if (lane == a) {
+        const int l = wave * kClusterSize + a;
+        lds_fix_add(&jacc[0][l], fix_from_float(gx));
+        lds_fix_add(&jacc[1][l], fix_from_float(gy));
+        lds_fix_add(&jacc[2][l], fix_from_float(gz));
+      }
     }
   }
   const double sev = wave_sum(ev);
   const double see = wave_sum(ee);
   const double svir = wave_sum(vir);
   if (lane == 0) {
-    e_partial[4 * ci + 0] = sev;
-    e_partial[4 * ci + 1] = see;
-    e_partial[4 * ci + 2] = svir;
-    e_partial[4 * ci + 3] = 0.0;
+    e_partial[4 * wid + 0] = sev;
+    e_partial[4 * wid + 1] = see;
+    e_partial[4 * wid + 2] = svir;
+    e_partial[4 * wid + 3] = 0.0;
+  }
+  __syncthreads();
+  // flush the touched slots of the candidate table, consecutive threads on
+  // consecutive slots of a j-cluster
+  const int ns = nr * kClusterSize;
+  for (int t = threadIdx.x; t < 3 * ns; t += blockDim.x) {
+    const int c = t / ns, l = t - c * ns;
+    const long long v = jacc[c][l];
+    if (v != 0) {
+      long long *f = c == 0 ? fx : (c == 1 ? fy : fz);
+      fix_add(&f[cand[l / kClusterSize] * kClusterSize + (l & (kClusterSize - 1))], v);
+    }
   }
 }
 
 template <int ELEC, bool VDW>
-void launch(Engine &E, int grid) {
-  hipLaunchKernelGGL((k_nonbonded<ELEC, VDW>), dim3(grid), dim3(64), 0, E.stream, E.nbk, E.n_clusters_d.p,
-                     grid, E.periodic ? E.box[0] : 0.0, E.periodic ? E.box[1] : 0.0,
-                     E.periodic ? E.box[2] : 0.0, E.sx.p, E.sy.p, E.sz.p, E.sq.p, E.sr.p, E.se.p, E.entries.p,
-                     (int)E.entry_capacity, E.cl_entry_count.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
+void launch(Engine &E) {
+  hipLaunchKernelGGL((k_nonbonded<ELEC, VDW>), dim3(E.max_groups), dim3(64 * kGroup), 0, E.stream, E.nbk,
+                     E.n_groups_d.p, E.grp_first.p, E.grp_count.p, E.wg_ncand.p, E.wg_cand.p,
+                     E.periodic ? E.box[0] : 0.0, E.periodic ? E.box[1] : 0.0, E.periodic ? E.box[2] : 0.0,
+                     E.spos.p, E.sprm.p, E.entries.p, (int)E.entry_capacity,
+                     E.cl_entry_count.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
 }
 
 }  // namespace
@@ -328,25 +398,21 @@ void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
 
 void Engine::gather_positions() {
   const int nslots = max_clusters * kClusterSize;
-  sx.resize(nslots);
-  sy.resize(nslots);
-  sz.resize(nslots);
+  spos.resize(nslots);
   hipLaunchKernelGGL(k_gather_slots, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots, n_clusters_d.p,
-                     slot_orig.p, x.p, y.p, z.p, sx.p, sy.p, sz.p);
+                     slot_orig.p, x.p, y.p, z.p, spos.p);
 }
 
 // Per-slot parameters and accumulators; called once per list rebuild.
 void Engine::prepare_slots() {
   const int nslots = max_clusters * kClusterSize;
-  sq.resize(nslots);
-  sr.resize(nslots);
-  se.resize(nslots);
+  sprm.resize(nslots);
   nbx.resize(nslots);
   nby.resize(nslots);
   nbz.resize(nslots);
   if (have_nb)
     hipLaunchKernelGGL(k_gather_params, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots,
-                       n_clusters_d.p, slot_orig.p, q_scaled.p, lj_r.p, lj_e.p, sq.p, sr.p, se.p);
+                       n_clusters_d.p, slot_orig.p, q_scaled.p, lj_r.p, lj_e.p, sprm.p);
 }
 
 void Engine::launch_nonbonded(double *energy_out) {
@@ -355,20 +421,20 @@ void Engine::launch_nonbonded(double *energy_out) {
   const int nslots = max_clusters * kClusterSize;
   hipLaunchKernelGGL(k_zero_ll, dim3((nslots + 255) / 256), dim3(256), 0, stream, nbx.p, nby.p, nbz.p, nslots);
   gather_positions();
-  const int grid = (max_clusters + 7) / 8 * 8;
-  e_partial.resize((size_t)grid * 4);
+  const int waves = max_groups * kGroup;
+  e_partial.resize((size_t)waves * 4);
   const bool v = nbk.vdw_on != 0;
   {
     ScopedTimer t(*this, "nonbonded");
     switch (nbk.elec_mode) {
-      case 0: v ? launch<EL_NONE, true>(*this, grid) : launch<EL_NONE, false>(*this, grid); break;
-      case 1: v ? launch<EL_PLAIN, true>(*this, grid) : launch<EL_PLAIN, false>(*this, grid); break;
-      case 3: v ? launch<EL_ESWITCH, true>(*this, grid) : launch<EL_ESWITCH, false>(*this, grid); break;
-      default: v ? launch<EL_FSWITCH, true>(*this, grid) : launch<EL_FSWITCH, false>(*this, grid); break;
+      case 0: v ? launch<EL_NONE, true>(*this) : launch<EL_NONE, false>(*this); break;
+      case 1: v ? launch<EL_PLAIN, true>(*this) : launch<EL_PLAIN, false>(*this); break;
+      case 3: v ? launch<EL_ESWITCH, true>(*this) : launch<EL_ESWITCH, false>(*this); break;
+      default: v ? launch<EL_FSWITCH, true>(*this) : launch<EL_FSWITCH, false>(*this); break;
     }
   }
   PDGPU_HIP(hipGetLastError());
-  reduce4(stream, e_partial.p, grid, energy_out);
+  reduce4(stream, e_partial.p, waves, energy_out);
 }
 
 }  // namespace pdgpu
