@@ -303,6 +303,7 @@ def main():
     te = time.perf_counter()
     nb_ms, nb_launch = eng.kernel_time("nonbonded")
     ps_ms, ps_launch = eng.kernel_time("pair_search")
+    pr_ms, pr_launch = eng.kernel_time("prune")
     eng.set_timing(False)
 
     wall = te - ts
@@ -356,6 +357,8 @@ def main():
             "avg_launch_us": round(ps_ms / max(ps_launch, 1) * 1e3, 3),
             "launches": ps_launch,
             "algorithmic_bytes": int(w.n * 24 + n_tot * 4),
+            "prune_avg_launch_us": round(pr_ms / max(pr_launch, 1) * 1e3, 3),
+            "prune_launches": pr_launch,
         },
     }
     if not args.no_gb:

@@ -254,6 +254,15 @@ struct Engine {
   DBuf<long long> nbx, nby, nbz;           // nonbonded force accumulators, slot order
   DBuf<uint2> entries;
   DBuf<int> cl_entry_start, cl_entry_count;
+  // pruned copy of the list: the entries with a pair within Cutoff + prune_margin
+  // at the prune positions pref; valid while every atom stays within
+  // prune_margin / 2 of pref (checked on the device each step)
+  DBuf<uint2> pentries;
+  DBuf<int> pcount;
+  DBuf<SlotPos> pref;
+  DBuf<unsigned> prune_dmax;               // max displacement from pref this step (float bits)
+  double prune_margin = 0.5;
+  bool prune_forced = true;
   DBuf<unsigned long long> entry_counter;  // [0] entries used, [1] overflow flag
   size_t entry_capacity = 0;
   int n_clusters = 0;                      // host copy after a build

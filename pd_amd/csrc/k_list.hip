@@ -609,6 +609,7 @@ void Engine::build_list() {
     entry_capacity = std::max(entry_capacity * 2, (size_t)(hc[1] * 1.25 + 64) / 64 * 64);
   }
   full_update_count++;
+  prune_forced = true;
   prepare_slots();
   list_valid = true;
 }

@@ -39,9 +39,17 @@ enum { EL_NONE = 0, EL_PLAIN = 1, EL_ESWITCH = 3, EL_FSWITCH = 4 };
 // sum); pairs within this band of a cutoff are re-tested in FP64.
 constexpr float kNearBand = 0.01f;
 
+// Image index (i+1)*9 + (j+1)*3 + (k+1) -> the basis vector (i*Lx, j*Ly, k*Lz)
+// of PeriodicBox::setupCellDimensions (space.cpp:94-109).
+__device__ __forceinline__ double image_shift(int img, int axis, double L) {
+  int k = axis == 0 ? img / 9 : (axis == 1 ? (img / 3) % 3 : img % 3);
+  return k == 1 ? 0.0 : (k == 2 ? L : -L);
+}
+
 __global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
                                const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, SlotPos *__restrict__ spos) {
+                               const double *__restrict__ z, SlotPos *__restrict__ spos,
+                               const SlotPos *__restrict__ pref, unsigned *__restrict__ dmax) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots || s >= *n_clusters * kClusterSize) return;
   int a = slot_orig[s];
@@ -50,6 +58,11 @@ __global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, c
     p.x = x[a];
     p.y = y[a];
     p.z = z[a];
+    // displacement since the last prune, rounded up (float bits order like the values)
+    const SlotPos r = pref[s];
+    const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+    const float d = (float)(sqrt(dx * dx + dy * dy + dz * dz) * (1.0 + 1e-6));
+    atomicMax(dmax, __float_as_uint(d == d ? d : 3.0e38f));
   } else {  // padding slot: far away, zero parameters, never listed
     p.x = 1.0e6;
     p.y = 1.0e6;
@@ -57,6 +70,64 @@ __global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, c
   }
   p.pad = 0.0;
   spos[s] = p;
+}
+
+// Rolling prune of the list: keeps the pairs within Cutoff + margin at the
+// current positions.  A pair left out is more than margin - 2 * dmax beyond the
+// cutoff at any later step, so while 2 * dmax stays below margin (minus a
+// rounding allowance) the pruned list holds every pair within the cutoff; the
+// kernel returns at once in that case.  One wave per i-cluster, entries kept in
+// their order.
+__global__ __launch_bounds__(64) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
+                                              float rk2, float margin_ok, int forced,
+                                              const unsigned *__restrict__ dmax,
+                                              const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
+                                              const int *__restrict__ cl_count, int stride,
+                                              uint2 *__restrict__ pentries, int *__restrict__ pcount,
+                                              SlotPos *__restrict__ pref) {
+  if (!forced && 2.0f * __uint_as_float(*dmax) <= margin_ok) return;
+  const int ci = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (ci >= *n_clusters) return;
+  if (lane < kClusterSize) pref[ci * kClusterSize + lane] = spos[ci * kClusterSize + lane];
+  const SlotPos oi = spos[ci * kClusterSize];
+  double iox[8], ioy[8], ioz[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const SlotPos p = spos[ci * kClusterSize + a];
+    iox[a] = p.x - oi.x;
+    ioy[a] = p.y - oi.y;
+    ioz[a] = p.z - oi.z;
+  }
+  const int cnt = cl_count[ci];
+  const uint2 *E = entries + (size_t)ci * stride;
+  uint2 *PE = pentries + (size_t)ci * stride;
+  int out = 0;
+  for (int k0 = 0; k0 < cnt; k0 += 64) {
+    const int k = k0 + lane;
+    uint2 ent = k < cnt ? E[k] : make_uint2(0u, 0u);
+    unsigned keep = 0u;
+    if (k < cnt) {
+      const int j = (int)(ent.x & 0xFFFFFFu);
+      const int img = (int)((ent.x >> 24) & 31u);
+      const SlotPos pj = spos[j];
+      const double ax = (pj.x - oi.x) + image_shift(img, 0, Lx), ay = (pj.y - oi.y) + image_shift(img, 1, Ly),
+                   az = (pj.z - oi.z) + image_shift(img, 2, Lz);
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        const float dx = (float)(ax - iox[a]), dy = (float)(ay - ioy[a]), dz = (float)(az - ioz[a]);
+        keep |= ((dx * dx + dy * dy + dz * dz) <= rk2 ? 1u : 0u) << a;
+      }
+      keep &= ent.y & 0xFFu;
+    }
+    const unsigned long long b = __ballot(keep != 0u);
+    if (keep) {
+      ent.y = (ent.y & ~0xFFu) | keep;
+      PE[out + __popcll(b & ((1ull << lane) - 1ull))] = ent;
+    }
+    out += __popcll(b);
+  }
+  if (lane == 0) pcount[ci] = out;
 }
 
 __global__ void k_gather_params(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
@@ -69,20 +140,14 @@ __global__ void k_gather_params(int nslots, const int *__restrict__ n_clusters, 
 }
 
 __global__ void k_zero_ll(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c,
-                          int count) {
+                          int count, unsigned *__restrict__ dmax) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *dmax = 0u;
   if (i < count) {
     a[i] = 0;
     b[i] = 0;
     c[i] = 0;
   }
-}
-
-// Image index (i+1)*9 + (j+1)*3 + (k+1) -> the basis vector (i*Lx, j*Ly, k*Lz)
-// of PeriodicBox::setupCellDimensions (space.cpp:94-109).
-__device__ __forceinline__ double image_shift(int img, int axis, double L) {
-  int k = axis == 0 ? img / 9 : (axis == 1 ? (img / 3) % 3 : img % 3);
-  return k == 1 ? 0.0 : (k == 2 ? L : -L);
 }
 
 __device__ __forceinline__ void lds_fix_add(long long *p, long long v) {
@@ -306,8 +371,8 @@ void launch(Engine &E) {
   hipLaunchKernelGGL((k_nonbonded<ELEC, VDW>), dim3(E.max_groups), dim3(64 * kGroup), 0, E.stream, E.nbk,
                      E.n_groups_d.p, E.grp_first.p, E.grp_count.p, E.wg_ncand.p, E.wg_cand.p,
                      E.periodic ? E.box[0] : 0.0, E.periodic ? E.box[1] : 0.0, E.periodic ? E.box[2] : 0.0,
-                     E.spos.p, E.sprm.p, E.entries.p, (int)E.entry_capacity,
-                     E.cl_entry_count.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
+                     E.spos.p, E.sprm.p, E.pentries.p, (int)E.entry_capacity,
+                     E.pcount.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
 }
 
 }  // namespace
@@ -399,8 +464,10 @@ void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
 void Engine::gather_positions() {
   const int nslots = max_clusters * kClusterSize;
   spos.resize(nslots);
+  pref.resize(nslots);
+  prune_dmax.resize(1);
   hipLaunchKernelGGL(k_gather_slots, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots, n_clusters_d.p,
-                     slot_orig.p, x.p, y.p, z.p, spos.p);
+                     slot_orig.p, x.p, y.p, z.p, spos.p, pref.p, prune_dmax.p);
 }
 
 // Per-slot parameters and accumulators; called once per list rebuild.
@@ -419,8 +486,21 @@ void Engine::launch_nonbonded(double *energy_out) {
   require(have_nb, "nonbonded component not set up");
   require(list_valid, "neighbour list not built");
   const int nslots = max_clusters * kClusterSize;
-  hipLaunchKernelGGL(k_zero_ll, dim3((nslots + 255) / 256), dim3(256), 0, stream, nbx.p, nby.p, nbz.p, nslots);
+  prune_dmax.resize(1);
+  hipLaunchKernelGGL(k_zero_ll, dim3((nslots + 255) / 256), dim3(256), 0, stream, nbx.p, nby.p, nbz.p, nslots,
+                     prune_dmax.p);
   gather_positions();
+  {
+    ScopedTimer t(*this, "prune");
+    pentries.resize((size_t)max_clusters * entry_capacity);
+    pcount.resize(max_clusters);
+    const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
+    hipLaunchKernelGGL(k_prune, dim3(max_clusters), dim3(64), 0, stream, n_clusters_d.p,
+                       periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
+                       (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p, spos.p,
+                       entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
+    prune_forced = false;
+  }
   const int waves = max_groups * kGroup;
   e_partial.resize((size_t)waves * 4);
   const bool v = nbk.vdw_on != 0;
