@@ -114,8 +114,11 @@ def build_chains(specs, workers):
 
 def gb_engine(G, s, seed):
     from pd_amd import units as U
+    from pd_amd import systems as S
     e = G.engine_for(s, None, bonded=True)
     e.setup_gb(G.gb_params(**GB_KW), s.gbrad, s.cov)
+    use, rad, P = S.lcpo_params(s)
+    e.setup_lcpo(G.LCPOParams(GlobalASP=0.005, UpdateSasa=5), rad, P, use)
     rng = np.random.default_rng(seed)
     e.set_velocities(rng.normal(size=(s.n, 3)) * np.sqrt(U.KB * 300.0 / s.mass)[:, None])
     return e
@@ -126,7 +129,7 @@ def gb_benchmarks(args, log):
     single-system engine looped over them, and 64 REMD replicas."""
     import torch
     from pd_amd import engine as G
-    comps = G.PDGPU_GB | G.PDGPU_BONDED
+    comps = G.PDGPU_GB | G.PDGPU_BONDED | G.PDGPU_SASA
     lang = dict(integrator=G.PDGPU_LANGEVIN, components=comps, timestep=2e-15, temperature=300.0,
                 fric_coeff=1e12, update_nlist=0, step_offset=0, total_steps=1 << 30, seed=3, noise_stream=0)
     relax = dict(lang, fric_coeff=5e13, temperature=100.0, timestep=1e-15)
@@ -146,7 +149,7 @@ def gb_benchmarks(args, log):
     e.md(G.MDParams(**dict(lang, step_offset=10)), args.gb_steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - ts
-    kt = {k: e.kernel_time(k) for k in ("gb_radii", "gb_pair", "gb_chain")}
+    kt = {k: e.kernel_time(k) for k in ("gb_radii", "gb_pair", "gb_chain", "lcpo")}
     e.set_timing(False)
     pair_us = kt["gb_pair"][0] / max(kt["gb_pair"][1], 1) * 1e3
     achieved = pairs * GB_FLOP_PAIR / (pair_us * 1e-6) / 1e12

@@ -83,6 +83,7 @@ typedef struct pdgpu_gb_params {
 #define PDGPU_NONBONDED 0x1u /* FF_NonBonded::calcForces */
 #define PDGPU_BONDED 0x2u    /* FF_Bonded bonds + angles + torsions */
 #define PDGPU_GB 0x4u        /* FF_GeneralizedBorn_Still::calcForces (FastMode) */
+#define PDGPU_SASA 0x8u      /* FF_SASA_LCPO::calcForces */
 
 /* Integrators of MolecularDynamics (md.h:76). */
 #define PDGPU_VELOCITY_VERLET 1
@@ -165,6 +166,23 @@ int pdgpu_set_bonded(pdgpu_engine *e, int32_t nb, const int32_t *bi, const int32
 int pdgpu_setup_gb(pdgpu_engine *e, const pdgpu_gb_params *p, const double *gbradius,
                    const int32_t *off12, const int32_t *idx12, const double *d12,
                    const int32_t *off13, const int32_t *idx13, const double *d13);
+
+/* FF_SASA_LCPO public parameters (lcpo.h:59-68). */
+typedef struct pdgpu_lcpo_params {
+  double GlobalASP;   /* atomic solvation parameter, kcal/mol/A^2 (lcpo.cpp:63) */
+  int32_t UpdateSasa; /* full recalculation every UpdateSasa MD steps (lcpo.cpp:196-201) */
+} pdgpu_lcpo_params;
+
+/* FF_SASA_LCPO::setup (lcpo.cpp:33-127) with the per-atom result of its
+ * LCPOSASATYPE lookup (readLCPOSection, lcpo.cpp:321-437): radius (the table
+ * radius; 1.4 A is added here), P1..P4 (4 per atom), use (0 for hydrogens).
+ * asp: per-atom solvation parameter in kcal/mol/A^2, or NULL for GlobalASP on
+ * every atom. */
+int pdgpu_setup_lcpo(pdgpu_engine *e, const pdgpu_lcpo_params *p, const double *radius, const double *P,
+                     const int8_t *use, const double *asp);
+
+/* Per-atom SASA (A^2) of the last full LCPO calculation (SASA_Atom::SASA). */
+int pdgpu_get_sasa(pdgpu_engine *e, double *sasa);
 
 /* ---- state ------------------------------------------------------------------ */
 

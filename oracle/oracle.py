@@ -131,6 +131,8 @@ def lib():
         L.pdo_langevin_noise.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _dp]
         L.pdo_remd_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
         L.pdo_remd_uniform.restype = C.c_double
+        L.pdo_lcpo_forces.argtypes = [C.c_int, _dp, _bp, _dp, _dp, _dp, _ip, _up, _dp, _dp]
+        L.pdo_lcpo_forces.restype = C.c_double
         _lib = L
     return _lib
 
@@ -341,6 +343,19 @@ def langevin_noise(seed, stream, step, atom) -> np.ndarray:
 
 def remd_uniform(seed, pair, rnd) -> float:
     return lib().pdo_remd_uniform(seed, pair, rnd)
+
+
+def lcpo_forces(pos, use, radius, P, sigma, counts, packed):
+    """FF_SASA_LCPO::calcLCPOSasaForces(true) (lcpo.cpp:643-962).  radius: LCPOSASATYPE
+    radii (1.4 A added inside), P: (n, 4) P1..P4, sigma: J/A^2 per atom.  Returns
+    (forces J/m, per-atom SASA A^2, cavity energy J)."""
+    n = len(radius)
+    f = np.zeros(3 * n)
+    sasa = np.zeros(n)
+    e = lib().pdo_lcpo_forces(n, _f64(pos).reshape(-1), np.ascontiguousarray(use, dtype=np.int8),
+                              _f64(radius), _f64(P).reshape(-1), _f64(sigma), _i32(counts),
+                              np.ascontiguousarray(packed, dtype=np.uint32), f, sasa)
+    return f.reshape(-1, 3), sasa, e
 
 
 def decode(packed):

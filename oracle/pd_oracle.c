@@ -1163,3 +1163,152 @@ double pdo_remd_uniform(uint64_t seed, uint32_t pair, uint32_t round) {
   pdo_philox4x32(ctr, key, w);
   return u53(w[0], w[1]);
 }
+
+/* ---- FF_SASA_LCPO ------------------------------------------------------------- */
+
+#define LCPO_NMAX 80 /* Nlistmax, lcpo.h:24 */
+
+/* calcLCPOSasaForces(true), lcpo.cpp:643-962, with the setup of lcpo.cpp:33-127:
+ * radius[i] is the LCPOSASATYPE radius (1.4 A is added here, lcpo.cpp:107),
+ * use[i] = 0 for hydrogens, P holds P1..P4 per atom, sigma is the atomic
+ * solvation parameter in J per A^2 per molecule (GlobalASP / Na / J2kcal,
+ * lcpo.cpp:63).  The overlap lists are filled from PD's packed neighbour list
+ * (j < i entries of atom i, in list order) and truncated at 80 entries in fill
+ * order.  Adds the forces (J/m) to f, writes the per-atom SASA (A^2) and
+ * returns the cavity energy (J). */
+double pdo_lcpo_forces(int n, const double *pos, const int8_t *use, const double *radius,
+                       const double *P, const double *sigma, const int32_t *counts,
+                       const uint32_t *packed, double *f, double *sasa) {
+  const double PI = PD_PI, TwoPI = 2.0 * PD_PI, FourPI = 4.0 * PD_PI;
+  int *nsz = (int *)calloc((size_t)n, sizeof(int));
+  int *nidx = (int *)malloc((size_t)n * LCPO_NMAX * sizeof(int));
+  double *ns = (double *)malloc((size_t)n * LCPO_NMAX * sizeof(double));
+  double *R = (double *)malloc((size_t)n * sizeof(double));
+  for (int i = 0; i < n; i++) R[i] = radius[i] + 1.4;
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const int64_t beg = off;
+    off += counts[i];
+    if (!use[i]) continue;
+    const double *pi = pos + 3 * i;
+    const double ri = R[i];
+    for (int64_t nj = beg; nj < beg + counts[i]; nj++) {
+      const int j = (int)(packed[nj] & 0xFFFFFFu);
+      if (j >= i) break;
+      if (!use[j]) continue;
+      const double *pj = pos + 3 * j;
+      const double g = pi[0] - pj[0], h = pi[1] - pj[1], k = pi[2] - pj[2];
+      double dij = g * g + h * h + k * k;
+      const double rj = R[j];
+      if (dij < sqr(ri + rj)) {
+        dij = sqrt(dij);
+        const double inv = 1 / dij;
+        const double t = 0.5 * (sqr(ri) - sqr(rj)) * inv;
+        const double sij = TwoPI * ri * (ri - dij * 0.5 - t);
+        const double sji = TwoPI * rj * (rj - dij * 0.5 + t);
+        if (nsz[i] < LCPO_NMAX) {
+          nidx[i * LCPO_NMAX + nsz[i]] = j;
+          ns[i * LCPO_NMAX + nsz[i]] = sij;
+          nsz[i]++;
+        }
+        if (nsz[j] < LCPO_NMAX) {
+          nidx[j * LCPO_NMAX + nsz[j]] = i;
+          ns[j * LCPO_NMAX + nsz[j]] = sji;
+          nsz[j]++;
+        }
+      }
+    }
+  }
+  double epot = 0.0;
+  for (int i = 0; i < n; i++) {
+    double der[3] = {0.0, 0.0, 0.0};
+    if (!use[i]) {
+      sasa[i] = 0.0;
+      continue;
+    }
+    const double *pi = pos + 3 * i;
+    const double ri = R[i];
+    const double Si = FourPI * ri * ri;
+    double Si2 = 0, Si3 = 0, Si4 = 0;
+    double a2[3] = {0, 0, 0}, a4[3] = {0, 0, 0};
+    for (int jn = 0; jn < nsz[i]; jn++) {
+      const int ja = nidx[i * LCPO_NMAX + jn];
+      const double *pj = pos + 3 * ja;
+      const double Si2_t = ns[i * LCPO_NMAX + jn];
+      double Si3_t = 0, Si3ik_t = 0;
+      const double gx = pj[0] - pi[0], gy = pj[1] - pi[1], gz = pj[2] - pi[2];
+      const double dij = sqrt(gx * gx + gy * gy + gz * gz);
+      const double rj = R[ja];
+      const double inv = 1 / dij;
+      double t = 0.5 * (sqr(ri) - sqr(rj)) * inv;
+      const double sji = TwoPI * rj * (rj - dij * 0.5 + t);
+      t *= 2.0 * inv;
+      double d3[3] = {0, 0, 0}, d4[3] = {0, 0, 0}, d3b[3] = {0, 0, 0}, d4b[3] = {0, 0, 0}, d2[3], a2t[3];
+      double dAdd = PI * rj * (-t - 1);
+      const double ddd[3] = {-(pi[0] - pj[0]) / dij, -(pi[1] - pj[1]) / dij, -(pi[2] - pj[2]) / dij};
+      for (int c = 0; c < 3; c++) d2[c] = dAdd * ddd[c];
+      dAdd = PI * ri * (t - 1);
+      for (int c = 0; c < 3; c++) a2t[c] = dAdd * ddd[c];
+      for (int kn = 0; kn < nsz[ja]; kn++) {
+        const int ka = nidx[ja * LCPO_NMAX + kn];
+        const double *pk = pos + 3 * ka;
+        const double rk = R[ka];
+        if (ka == i) continue;
+        const double hx = pk[0] - pi[0], hy = pk[1] - pi[1], hz = pk[2] - pi[2];
+        double dik = hx * hx + hy * hy + hz * hz;
+        if (sqr(ri + rk) < dik) continue;
+        dik = sqrt(dik);
+        const double invik = 1 / dik;
+        const double u = (sqr(ri) - sqr(rk)) * sqr(invik);
+        double dA = PI * ri * (u - 1);
+        const double dk[3] = {-(pi[0] - pk[0]) * invik, -(pi[1] - pk[1]) * invik, -(pi[2] - pk[2]) * invik};
+        const double sjk = ns[ja * LCPO_NMAX + kn];
+        for (int c = 0; c < 3; c++) d3[c] += dA * dk[c];
+        dA = PI * rk * (-u - 1);
+        Si3ik_t += PI * ri * (2 * ri - dik - dik * u);
+        for (int c = 0; c < 3; c++) d3b[c] += dA * dk[c];
+        for (int c = 0; c < 3; c++) d4b[c] += sjk * dA * dk[c];
+        Si3_t += sjk;
+      }
+      for (int c = 0; c < 3; c++) d4[c] = sji * d3[c] + d4b[c];
+      for (int c = 0; c < 3; c++) d3[c] += d3b[c];
+      for (int c = 0; c < 3; c++) d4[c] += d2[c] * Si3ik_t;
+      const double *Pj = P + 4 * ja;
+      for (int c = 0; c < 3; c++) {
+        d2[c] *= Pj[1] * sigma[ja];
+        d3[c] *= Pj[2] * sigma[ja];
+        d4[c] *= Pj[3] * sigma[ja];
+        der[c] += d2[c];
+        der[c] += d3[c];
+        der[c] += d4[c];
+      }
+      Si4 += (Si3_t * Si2_t);
+      Si3 += Si3_t;
+      Si2 += Si2_t;
+      for (int c = 0; c < 3; c++) {
+        a2[c] += a2t[c];
+        a4[c] += Si3_t * a2t[c];
+      }
+    }
+    const double *Pi = P + 4 * i;
+    double SAi = Pi[0] * Si + Pi[1] * Si2 + Pi[2] * Si3 + Pi[3] * Si4;
+    for (int c = 0; c < 3; c++) {
+      a2[c] *= Pi[1] * sigma[i];
+      a4[c] *= Pi[3] * sigma[i];
+      der[c] += a2[c];
+      der[c] += a4[c];
+    }
+    if (SAi < 0) {
+      SAi = 0;
+      der[0] = der[1] = der[2] = 0.0;
+    }
+    sasa[i] = SAi;
+    epot += SAi * sigma[i];
+    for (int c = 0; c < 3; c++) f[3 * i + c] += der[c] * (1 / Angstrom);
+  }
+  free(nsz);
+  free(nidx);
+  free(ns);
+  free(R);
+  return epot;
+}

@@ -196,6 +196,15 @@ void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
  * (r, round, 0x52454D44, 0) with key `seed`, 53 bits mapped to (0,1]. */
 double pdo_remd_uniform(uint64_t seed, uint32_t pair, uint32_t round);
 
+/* FF_SASA_LCPO::calcLCPOSasaForces(true) (lcpo.cpp:643-962): LCPO SASA from the
+ * overlap lists built out of PD's packed neighbour list (80 entries at most,
+ * fill order).  radius = LCPOSASATYPE radius (1.4 added inside), P = P1..P4 per
+ * atom, sigma in J/A^2.  Adds forces (J/m), writes per-atom SASA, returns the
+ * cavity energy (J). */
+double pdo_lcpo_forces(int n, const double *pos, const int8_t *use, const double *radius,
+                       const double *P, const double *sigma, const int32_t *counts,
+                       const uint32_t *packed, double *f, double *sasa);
+
 #ifdef __cplusplus
 }
 #endif

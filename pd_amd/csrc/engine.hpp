@@ -62,6 +62,7 @@ struct Consts {
   static double halfeconv_joule() { return econv_joule() * 0.5; }
   // kcal/mol -> J per molecule; kcal/mol/A -> J/m
   static double kcal_to_J() { return kcal2J / Na; }
+  static double J_to_kcalmol() { return J2kcal() * Na; }
   static double kcalA_to_Jm() { return kcal2J / Na / Angstrom; }
 };
 
@@ -199,6 +200,28 @@ struct GBState {
            long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner);
 };
 
+// ---- LCPO SASA (k_lcpo.hip) ------------------------------------------------------
+constexpr int kLcpoMax = 80;  // Nlistmax, lcpo.h:24
+struct LCPOState {
+  int n = 0;
+  int update = 5;  // FF_SASA_LCPO::UpdateSasa (lcpo.h:67)
+  std::vector<int8_t> h_use;
+  std::vector<double> h_R, h_P, h_sig;  // radius + 1.4 A, P1..P4, sigma (J/A^2 per molecule)
+  std::vector<int> h_atom_sys, h_first, h_count;
+  DBuf<int8_t> use;
+  DBuf<double> R, sig, nl_s, sasa, dvx, dvy, dvz, eatom;
+  DBuf<double4> P;
+  DBuf<int> atom_sys, sys_first, sys_count, nl_cnt, nl_idx;
+  void add_system(int first, int count, const int8_t *use, const double *radius, const double *P4,
+                  const double *sigma_j);
+  void append(const LCPOState &o, int first);  // another system's atoms, as they are
+  void upload(hipStream_t s);
+  // full: recompute now; otherwise recompute when step % update == 0 (step from
+  // step_ptr on the device when given, else step_host) and re-apply otherwise
+  void run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
+           long long *fz, const int *step_ptr, int step_host, bool full);
+};
+
 void gb_fixed_terms(int n, const double *gbrad, const pdgpu_gb_params &p, const int32_t *off12,
                     const int32_t *idx12, const double *d12, const int32_t *off13, const int32_t *idx13,
                     const double *d13, std::vector<double> &Vj, std::vector<double> &terms123);
@@ -311,11 +334,15 @@ struct Engine {
   // integrate.hip
   void zero_forces();
   void forces_to_host(double *f, size_t stride_bytes);
-  void evaluate(uint32_t components);        // forces + energies into e_out
+  void evaluate(uint32_t components, int step = -1);  // forces + energies into e_out (step < 0: LCPO in full)
   void energies_to_host(uint32_t components, pdgpu_energies *out);
   void integrate(const pdgpu_md_params &p, int step, double *ekin_slot);
   void md(const pdgpu_md_params &p, int n_steps, double *epot_trace, double *ekin_trace);
   uint32_t last_components = 0;
+  // k_lcpo.hip
+  std::unique_ptr<LCPOState> lcpo;
+  DBuf<double> e_partial_lcpo;
+  void launch_lcpo(double *energy_slot, int step, bool full);
   // k_gb.hip
   bool have_gb = false;
   std::unique_ptr<GBState> gb;

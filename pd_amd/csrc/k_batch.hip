@@ -25,7 +25,7 @@ namespace pdgpu {
 namespace {
 
 constexpr int kStride = 12;  // per-system energy slots, kcal/mol except [8]
-// [0] GB cross  [1] vdW  [2] elec  [3] -  [4] GB self  [5] bond  [6] angle
+// [0] GB cross  [1] vdW  [2] elec  [3] LCPO cavity  [4] GB self  [5] bond  [6] angle
 // [7] torsion  [8] kinetic (J)  [9] epot (J)
 
 __global__ void k_zero3n(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c, int n) {
@@ -167,7 +167,7 @@ __global__ void k_batch_record(int nsys, double *__restrict__ e_sys, double kcal
   const int local = counters[1];
   if (s < nsys) {
     double *e = e_sys + (size_t)s * kStride;
-    const double ep = (e[0] + e[1] + e[2] + e[4] + e[5] + e[6] + e[7]) * kcal_to_J;
+    const double ep = (e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6] + e[7]) * kcal_to_J;
     e[9] = ep;
     trace[((size_t)local * nsys + s) * 2 + 0] = ep;
     trace[((size_t)local * nsys + s) * 2 + 1] = e[8];
@@ -227,6 +227,7 @@ struct Batch {
   DBuf<int> atom_sys, sys_first;
   std::unique_ptr<GBState> gb;
   bool have_gb = false;
+  std::unique_ptr<LCPOState> lcpo;  // when every member has an LCPO component
   // bonded terms, global atom indices, numbered bonds | angles | torsions
   DBuf<int> bi, bj, ai, aa, aj, ti, ta, tb, tj, tterms;
   DBuf<double> bl, bk, at0, ak, tvn, tnn, tgamma;
@@ -268,6 +269,11 @@ struct Batch {
       for (int c = 0; c < 3; c++)
         hipLaunchKernelGGL(k_seg_sum, dim3(nsys), dim3(256), 0, stream, eterm.p, seg_beg[c].p, seg_end[c].p,
                            e_sys.p, kStride, 5 + c);
+    }
+    if ((comps & PDGPU_SASA) && lcpo) {
+      lcpo->run(stream, x.p, y.p, z.p, fx.p, fy.p, fz.p, counters.p, 0, false);
+      hipLaunchKernelGGL(k_seg_sum, dim3(nsys), dim3(256), 0, stream, lcpo->eatom.p, atom_beg.p, atom_end.p, e_sys.p,
+                         kStride, 3);
     }
     hipLaunchKernelGGL(k_batch_integrate, dim3((natoms + T - 1) / T), dim3(T), 0, stream, natoms, L, counters.p,
                        atom_sys.p, sys_first.p, temps.p, mass.p, fx.p, fy.p, fz.p, x.p, y.p, z.p, vx.p, vy.p, vz.p,
@@ -398,6 +404,8 @@ int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_ba
     std::vector<double> t123;
     auto gb = std::make_unique<pdgpu::GBState>();
     bool all_gb = true;
+    auto lc = std::make_unique<pdgpu::LCPOState>();
+    bool all_lcpo = true;
     for (int s = 0; s < n_systems; s++) {
       Engine *E = reinterpret_cast<Engine *>(systems[s]);
       if (!E || !E->have_positions || E->mass.n < (size_t)E->n) {
@@ -461,6 +469,12 @@ int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_ba
       } else {
         all_gb = false;
       }
+      if (E->lcpo) {
+        if (s == 0) lc->update = E->lcpo->update;
+        lc->append(*E->lcpo, off);
+      } else {
+        all_lcpo = false;
+      }
       E->sync();
       B->natoms += n;
     }
@@ -507,6 +521,10 @@ int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_ba
       gb->gbrad.upload(gbr, st); gb->Vj.upload(vj, st); gb->terms123.upload(t123, st);
       b.gb = std::move(gb);
       b.have_gb = true;
+    }
+    if (all_lcpo) {
+      lc->upload(st);
+      b.lcpo = std::move(lc);
     }
     b.zero_energies();
     b.sync();
@@ -596,6 +614,11 @@ int pdgpu_batch_compute(pdgpu_batch *h, uint32_t components, double *f_accum, pd
                        b.natoms);
     if ((components & PDGPU_GB) && b.have_gb)
       b.gb->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.e_sys.p, pdgpu::kStride, 0, nullptr);
+    if ((components & PDGPU_SASA) && b.lcpo) {
+      b.lcpo->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, nullptr, 0, true);
+      hipLaunchKernelGGL(pdgpu::k_seg_sum, dim3(b.nsys), dim3(256), 0, b.stream, b.lcpo->eatom.p, b.atom_beg.p,
+                         b.atom_end.p, b.e_sys.p, pdgpu::kStride, 3);
+    }
     if ((components & PDGPU_BONDED) && b.nb + b.na + b.nt > 0) {
       pdgpu::BondedArrays B{b.nb, b.na, b.nt, b.bi.p, b.bj.p, b.bl.p, b.bk.p, b.ai.p, b.aa.p, b.aj.p, b.at0.p,
                             b.ak.p, b.ti.p, b.ta.p, b.tb.p, b.tj.p, b.tterms.p, b.tvn.p, b.tnn.p, b.tgamma.p};
@@ -628,7 +651,8 @@ int pdgpu_batch_compute(pdgpu_batch *h, uint32_t components, double *f_accum, pd
         o.epot_bond = v[5] * kj;
         o.epot_angle = v[6] * kj;
         o.epot_torsion = v[7] * kj;
-        o.epot = (v[0] + v[1] + v[2] + v[4] + v[5] + v[6] + v[7]) * kj;
+        o.epot_surf = v[3] * kj;
+        o.epot = (v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7]) * kj;
         energies[s] = o;
       }
     if (f_accum) {

@@ -219,10 +219,11 @@ __global__ __launch_bounds__(256) void k_langevin(
 
 // trace[2*s] = epot (J), trace[2*s+1] = ekin (J); energies are in kcal/mol
 // in e_out (4 doubles per component slot).
-__global__ void k_record(const double *__restrict__ e_out, int use_nb, int use_bonded, int use_gb, double vdwcor,
-                         double kcal_to_J, double *__restrict__ trace, int s) {
+__global__ void k_record(const double *__restrict__ e_out, int use_nb, int use_bonded, int use_gb, int use_sasa,
+                         double vdwcor, double kcal_to_J, double *__restrict__ trace, int s) {
   if (threadIdx.x != 0) return;
   double ep = 0.0;
+  if (use_sasa) ep += e_out[13];
   if (use_nb) ep += e_out[0] + e_out[1] + vdwcor;
   if (use_bonded) ep += e_out[4] + e_out[5] + e_out[6];
   if (use_gb) ep += e_out[8] + e_out[9] + e_out[10] + e_out[12];
@@ -238,7 +239,7 @@ void reduce4(hipStream_t s, const double *part, int count, double *out) {
 
 // e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
 // bond, angle, torsion, virial; [8..11] GB pol, vdw, elec, virial (see k_gb.hip);
-// [12] GB self; [16] kinetic energy (J); others spare.
+// [12] GB self; [13] LCPO cavity energy; [16] kinetic energy (J); others spare.
 void Engine::zero_forces() {
   fbx.resize(n);
   fby.resize(n);
@@ -246,7 +247,7 @@ void Engine::zero_forces() {
   hipLaunchKernelGGL(k_zero3, dim3((n + 255) / 256), dim3(256), 0, stream, fbx.p, fby.p, fbz.p, n);
 }
 
-void Engine::evaluate(uint32_t components) {
+void Engine::evaluate(uint32_t components, int step) {
   e_out.resize(24);
   PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 24 * sizeof(double), stream));
   zero_forces();
@@ -256,6 +257,7 @@ void Engine::evaluate(uint32_t components) {
   }
   if ((components & PDGPU_BONDED) && (nbonds + nangles + ntors) > 0) launch_bonded(e_out.p + 4);
   if (components & PDGPU_GB) launch_gb(e_out.p + 8);
+  if (components & PDGPU_SASA) launch_lcpo(e_out.p + 13, step, step < 0);
   last_components = components;
 }
 
@@ -315,6 +317,10 @@ void Engine::energies_to_host(uint32_t components, pdgpu_energies *out) {
     e.epot += (h[4] + h[5] + h[6]) * kj;
   }
   if (components & PDGPU_GB) gb_energies_to_host(h + 8, e);
+  if (components & PDGPU_SASA) {
+    e.epot_surf += h[13] * kj;
+    e.epot += h[13] * kj;
+  }
   *out = e;
 }
 
@@ -398,11 +404,11 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
   for (int s = 0; s < n_steps; s++) {
     const int step = p.step_offset + s;
     if ((comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0) build_list();
-    evaluate(comp);
+    evaluate(comp, step);
     integrate(p, step, e_out.p + 16);
     hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, stream, e_out.p, (comp & PDGPU_NONBONDED) ? 1 : 0,
-                       (comp & PDGPU_BONDED) ? 1 : 0, (comp & PDGPU_GB) ? 1 : 0, cor, Consts::kcal_to_J(),
-                       trace.p, s);
+                       (comp & PDGPU_BONDED) ? 1 : 0, (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor,
+                       Consts::kcal_to_J(), trace.p, s);
   }
   std::vector<double> h((size_t)n_steps * 2);
   trace.download(h.data(), h.size(), stream);

@@ -1,0 +1,354 @@
+// k_lcpo.hip -- FF_SASA_LCPO (LCPO solvent-accessible surface) on the device.
+//
+// Restates FF_SASA_LCPO::calcLCPOSasaForces (lcpo.cpp:643-962), the linear
+// combination of pairwise overlaps of Weiser, Shenkin & Still:
+//   SASA_i = P1 S_i + P2 sum_j A_ij + P3 sum_j sum_k A_jk + P4 sum_j A_ij sum_k A_jk
+// over the overlap neighbours j of i and the neighbours k of j that also
+// overlap i, with S_i = 4 pi R_i^2, R = LCPO radius + 1.4 A, and the energy
+// sum_i sigma_i SASA_i (lcpo.cpp:63).  Three kernels:
+//   k_lcpo_lists  one wavefront per atom: the overlap partners (d < R_i + R_j,
+//                 hydrogens excluded) in ascending index order, at most 80
+//                 (Nlistmax, lcpo.h:24) -- the order and truncation PD's fill
+//                 loop produces (lcpo.cpp:713-767), and the overlap
+//                 A_ij = 2 pi R_i (R_i - d/2 - (R_i^2 - R_j^2) / 2d);
+//   k_lcpo_sasa   one thread per atom: SASA_i and the derivative that PD
+//                 collects on atom i (its own self terms plus the derivatives
+//                 of its neighbours' SASA with respect to x_i), clamped to zero
+//                 with the SASA when SASA_i < 0 (lcpo.cpp:940-944);
+//   k_lcpo_apply  every step: adds the stored derivative to the force
+//                 accumulators and writes sigma_i SASA_i.
+// PD recomputes only every UpdateSasa steps and re-applies the stored forces in
+// between (lcpo.cpp:196-201, 659-677); the first two kernels return at once on
+// the other steps (the step is read on the device, so a captured graph replays
+// correctly).  Overlap partners are searched over all atoms of the atom's
+// system, which is PD's neighbour list whenever that list holds every pair
+// within R_i + R_j (the all-pairs GB/SA setting, or any list of at least 7 A,
+// lcpo.cpp:37).  Everything is FP64 in PD's expression order, so the SASA and
+// forces reproduce the CPU arithmetic.
+#include "engine.hpp"
+
+#include <cmath>
+
+namespace pdgpu {
+
+namespace {
+
+__device__ __forceinline__ bool lcpo_full(const int *step_ptr, int step_host, int update) {
+  if (update <= 0) return true;
+  const int step = step_ptr ? *step_ptr : step_host;
+  return (step % update) == 0;
+}
+
+__global__ __launch_bounds__(64) void k_lcpo_lists(int n, const int *__restrict__ atom_sys,
+                                                   const int *__restrict__ sys_first,
+                                                   const int *__restrict__ sys_count, const double *__restrict__ x,
+                                                   const double *__restrict__ y, const double *__restrict__ z,
+                                                   const int8_t *__restrict__ use, const double *__restrict__ R,
+                                                   const int *__restrict__ step_ptr, int step_host, int update,
+                                                   int *__restrict__ nl_cnt, int *__restrict__ nl_idx,
+                                                   double *__restrict__ nl_s) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (i >= n || !lcpo_full(step_ptr, step_host, update)) return;
+  if (!use[i]) {
+    if (lane == 0) nl_cnt[i] = 0;
+    return;
+  }
+  const int s = atom_sys[i];
+  const int first = sys_first[s], count = sys_count[s];
+  const double xi = x[i], yi = y[i], zi = z[i], ri = R[i];
+  const double TwoPI = 2.0 * Consts::PI;
+  int cnt = 0;
+  for (int base = 0; base < count && cnt < kLcpoMax; base += 64) {
+    const int j = first + base + lane;
+    bool ok = false;
+    double d2 = 0.0, rj = 0.0;
+    if (base + lane < count && j != i && use[j]) {
+      const double g = xi - x[j], h = yi - y[j], k = zi - z[j];
+      d2 = g * g + h * h + k * k;
+      rj = R[j];
+      ok = d2 < (ri + rj) * (ri + rj);
+    }
+    const unsigned long long b = __ballot(ok);
+    const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
+    if (ok && pos < kLcpoMax) {
+      const double d = sqrt(d2);
+      const double inv = 1 / d;
+      const double t = 0.5 * (ri * ri - rj * rj) * inv;
+      nl_idx[(size_t)i * kLcpoMax + pos] = j;
+      nl_s[(size_t)i * kLcpoMax + pos] = TwoPI * ri * (ri - d * 0.5 - t);
+    }
+    cnt += __popcll(b);
+  }
+  if (lane == 0) nl_cnt[i] = min(cnt, kLcpoMax);
+}
+
+__global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ z, const int8_t *__restrict__ use,
+                                                  const double *__restrict__ R, const double4 *__restrict__ P,
+                                                  const double *__restrict__ sig, const int *__restrict__ step_ptr,
+                                                  int step_host, int update, const int *__restrict__ nl_cnt,
+                                                  const int *__restrict__ nl_idx, const double *__restrict__ nl_s,
+                                                  double *__restrict__ sasa, double *__restrict__ dvx,
+                                                  double *__restrict__ dvy, double *__restrict__ dvz) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !lcpo_full(step_ptr, step_host, update)) return;
+  const double PI = Consts::PI, TwoPI = 2.0 * Consts::PI, FourPI = 4.0 * Consts::PI;
+  double der[3] = {0.0, 0.0, 0.0};
+  if (!use[i]) {
+    sasa[i] = 0.0;
+    dvx[i] = dvy[i] = dvz[i] = 0.0;
+    return;
+  }
+  const double pi[3] = {x[i], y[i], z[i]};
+  const double ri = R[i];
+  const double Si = FourPI * ri * ri;
+  double Si2 = 0, Si3 = 0, Si4 = 0;
+  double a2[3] = {0, 0, 0}, a4[3] = {0, 0, 0};
+  const int ni = nl_cnt[i];
+  for (int jn = 0; jn < ni; jn++) {
+    const int ja = nl_idx[(size_t)i * kLcpoMax + jn];
+    const double pj[3] = {x[ja], y[ja], z[ja]};
+    const double Si2_t = nl_s[(size_t)i * kLcpoMax + jn];
+    double Si3_t = 0, Si3ik_t = 0;
+    const double gx = pj[0] - pi[0], gy = pj[1] - pi[1], gz = pj[2] - pi[2];
+    const double dij = sqrt(gx * gx + gy * gy + gz * gz);
+    const double rj = R[ja];
+    const double inv = 1 / dij;
+    double t = 0.5 * (ri * ri - rj * rj) * inv;
+    const double sji = TwoPI * rj * (rj - dij * 0.5 + t);
+    t *= 2.0 * inv;
+    double d3[3] = {0, 0, 0}, d4[3], d3b[3] = {0, 0, 0}, d4b[3] = {0, 0, 0}, d2[3], a2t[3];
+    double dAdd = PI * rj * (-t - 1);
+    const double ddd[3] = {-(pi[0] - pj[0]) / dij, -(pi[1] - pj[1]) / dij, -(pi[2] - pj[2]) / dij};
+    for (int c = 0; c < 3; c++) d2[c] = dAdd * ddd[c];
+    dAdd = PI * ri * (t - 1);
+    for (int c = 0; c < 3; c++) a2t[c] = dAdd * ddd[c];
+    const int nj = nl_cnt[ja];
+    for (int kn = 0; kn < nj; kn++) {
+      const int ka = nl_idx[(size_t)ja * kLcpoMax + kn];
+      if (ka == i) continue;
+      const double pk[3] = {x[ka], y[ka], z[ka]};
+      const double rk = R[ka];
+      const double hx = pk[0] - pi[0], hy = pk[1] - pi[1], hz = pk[2] - pi[2];
+      double dik = hx * hx + hy * hy + hz * hz;
+      if ((ri + rk) * (ri + rk) < dik) continue;
+      dik = sqrt(dik);
+      const double invik = 1 / dik;
+      const double u = (ri * ri - rk * rk) * (invik * invik);
+      double dA = PI * ri * (u - 1);
+      const double dk[3] = {-(pi[0] - pk[0]) * invik, -(pi[1] - pk[1]) * invik, -(pi[2] - pk[2]) * invik};
+      const double sjk = nl_s[(size_t)ja * kLcpoMax + kn];
+      for (int c = 0; c < 3; c++) d3[c] += dA * dk[c];
+      dA = PI * rk * (-u - 1);
+      Si3ik_t += PI * ri * (2 * ri - dik - dik * u);
+      for (int c = 0; c < 3; c++) d3b[c] += dA * dk[c];
+      for (int c = 0; c < 3; c++) d4b[c] += sjk * dA * dk[c];
+      Si3_t += sjk;
+    }
+    for (int c = 0; c < 3; c++) d4[c] = sji * d3[c] + d4b[c];
+    for (int c = 0; c < 3; c++) d3[c] += d3b[c];
+    for (int c = 0; c < 3; c++) d4[c] += d2[c] * Si3ik_t;
+    const double4 Pj = P[ja];
+    const double sj = sig[ja];
+    for (int c = 0; c < 3; c++) {
+      d2[c] *= Pj.y * sj;
+      d3[c] *= Pj.z * sj;
+      d4[c] *= Pj.w * sj;
+      der[c] += d2[c];
+      der[c] += d3[c];
+      der[c] += d4[c];
+    }
+    Si4 += (Si3_t * Si2_t);
+    Si3 += Si3_t;
+    Si2 += Si2_t;
+    for (int c = 0; c < 3; c++) {
+      a2[c] += a2t[c];
+      a4[c] += Si3_t * a2t[c];
+    }
+  }
+  const double4 Pi = P[i];
+  double SAi = Pi.x * Si + Pi.y * Si2 + Pi.z * Si3 + Pi.w * Si4;
+  for (int c = 0; c < 3; c++) {
+    a2[c] *= Pi.y * sig[i];
+    a4[c] *= Pi.w * sig[i];
+    der[c] += a2[c];
+    der[c] += a4[c];
+  }
+  if (SAi < 0) {  // LCPO can go negative for buried atoms (lcpo.cpp:940-944)
+    SAi = 0;
+    der[0] = der[1] = der[2] = 0.0;
+  }
+  sasa[i] = SAi;
+  dvx[i] = der[0];
+  dvy[i] = der[1];
+  dvz[i] = der[2];
+}
+
+// Adds the stored derivatives (J/A per molecule) as forces in fixed point
+// (kcal/mol/A) and writes each atom's sigma * SASA in kcal/mol.
+__global__ void k_lcpo_apply(int n, double to_kcal, const double *__restrict__ sasa, const double *__restrict__ sig,
+                             const double *__restrict__ dvx, const double *__restrict__ dvy,
+                             const double *__restrict__ dvz, long long *__restrict__ fx, long long *__restrict__ fy,
+                             long long *__restrict__ fz, double *__restrict__ eatom) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  eatom[i] = sasa[i] * sig[i] * to_kcal;
+  const double gx = dvx[i], gy = dvy[i], gz = dvz[i];
+  if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
+    fix_add(&fx[i], fix_from_double(gx * to_kcal));
+    fix_add(&fy[i], fix_from_double(gy * to_kcal));
+    fix_add(&fz[i], fix_from_double(gz * to_kcal));
+  }
+}
+
+}  // namespace
+
+void LCPOState::add_system(int first, int count, const int8_t *use, const double *radius, const double *P4,
+                           const double *sigma_j) {
+  for (int a = 0; a < count; a++) {
+    h_use.push_back(use[a] ? 1 : 0);
+    h_R.push_back(radius[a] + 1.4);  // FF_SASA_LCPO::setup, lcpo.cpp:107
+    for (int c = 0; c < 4; c++) h_P.push_back(P4[4 * a + c]);
+    h_sig.push_back(sigma_j[a]);
+    h_atom_sys.push_back((int)h_first.size());
+  }
+  h_first.push_back(first);
+  h_count.push_back(count);
+  n = (int)h_use.size();
+}
+
+void LCPOState::append(const LCPOState &o, int first) {
+  const int sys = (int)h_first.size();
+  h_use.insert(h_use.end(), o.h_use.begin(), o.h_use.end());
+  h_R.insert(h_R.end(), o.h_R.begin(), o.h_R.end());
+  h_P.insert(h_P.end(), o.h_P.begin(), o.h_P.end());
+  h_sig.insert(h_sig.end(), o.h_sig.begin(), o.h_sig.end());
+  h_atom_sys.insert(h_atom_sys.end(), o.n, sys);
+  h_first.push_back(first);
+  h_count.push_back(o.n);
+  n = (int)h_use.size();
+}
+
+void LCPOState::upload(hipStream_t s) {
+  use.upload(h_use, s);
+  R.upload(h_R, s);
+  std::vector<double4> p4(n);
+  for (int a = 0; a < n; a++) p4[a] = make_double4(h_P[4 * a], h_P[4 * a + 1], h_P[4 * a + 2], h_P[4 * a + 3]);
+  P.upload(p4, s);
+  sig.upload(h_sig, s);
+  atom_sys.upload(h_atom_sys, s);
+  sys_first.upload(h_first, s);
+  sys_count.upload(h_count, s);
+  nl_cnt.resize(n);
+  nl_idx.resize((size_t)n * kLcpoMax);
+  nl_s.resize((size_t)n * kLcpoMax);
+  sasa.resize(n);
+  dvx.resize(n);
+  dvy.resize(n);
+  dvz.resize(n);
+  eatom.resize(n);
+  std::vector<double> zero(n, 0.0);  // nothing to re-apply before the first full step
+  sasa.upload(zero, s);
+  dvx.upload(zero, s);
+  dvy.upload(zero, s);
+  dvz.upload(zero, s);
+}
+
+void LCPOState::run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
+                    long long *fz, const int *step_ptr, int step_host, bool full) {
+  const int upd = full ? 0 : update;
+  hipLaunchKernelGGL(k_lcpo_lists, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z, use.p,
+                     R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
+  hipLaunchKernelGGL(k_lcpo_sasa, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p, step_ptr,
+                     step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
+  hipLaunchKernelGGL(k_lcpo_apply, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p, sig.p,
+                     dvx.p, dvy.p, dvz.p, fx, fy, fz, eatom.p);
+  PDGPU_HIP(hipGetLastError());
+}
+
+}  // namespace pdgpu
+
+namespace pdgpu {
+
+void reduce4(hipStream_t s, const double *part, int count, double *out);
+
+namespace {
+// Sums per-atom energies into per-block partials (4 per block, first slot used).
+__global__ void k_block_sum(int n, const double *__restrict__ v, double *__restrict__ part) {
+  __shared__ double s[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  s[threadIdx.x] = i < n ? v[i] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = threadIdx.x == 0 ? s[0] : 0.0;
+}
+}  // namespace
+
+void Engine::launch_lcpo(double *energy_slot, int step, bool full) {
+  require(lcpo != nullptr, "LCPO component not set up");
+  {
+    ScopedTimer t(*this, "lcpo");
+    lcpo->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, nullptr, step, full);
+  }
+  const int blocks = (n + 255) / 256;
+  e_partial_lcpo.resize((size_t)blocks * 4);
+  hipLaunchKernelGGL(k_block_sum, dim3(blocks), dim3(256), 0, stream, n, lcpo->eatom.p, e_partial_lcpo.p);
+  reduce4(stream, e_partial_lcpo.p, blocks, energy_slot);
+}
+
+}  // namespace pdgpu
+
+using pdgpu::Engine;
+
+extern "C" int pdgpu_setup_lcpo(pdgpu_engine *h, const pdgpu_lcpo_params *p, const double *radius, const double *P,
+                                const int8_t *use, const double *asp) {
+  Engine *E = reinterpret_cast<Engine *>(h);
+  if (!E) return PDGPU_ERR_ARG;
+  if (!p || !radius || !P || !use) {
+    E->last_error = "pdgpu_setup_lcpo: null argument";
+    return PDGPU_ERR_ARG;
+  }
+  try {
+    PDGPU_HIP(hipSetDevice(E->device));
+    auto L = std::make_unique<pdgpu::LCPOState>();
+    L->update = p->UpdateSasa;
+    // sigma = ASP / Na / J2kcal: J per A^2 per molecule (lcpo.cpp:63)
+    std::vector<double> sig(E->n);
+    for (int a = 0; a < E->n; a++)
+      sig[a] = (asp ? asp[a] : p->GlobalASP) / pdgpu::Consts::Na / pdgpu::Consts::J2kcal();
+    L->add_system(0, E->n, use, radius, P, sig.data());
+    L->upload(E->stream);
+    E->sync();
+    E->lcpo = std::move(L);
+    return PDGPU_OK;
+  } catch (const pdgpu::EngineError &x) {
+    E->last_error = x.what();
+    return x.code;
+  } catch (const std::exception &x) {
+    E->last_error = x.what();
+    return PDGPU_ERR_SETUP;
+  }
+}
+
+extern "C" int pdgpu_get_sasa(pdgpu_engine *h, double *sasa) {
+  Engine *E = reinterpret_cast<Engine *>(h);
+  if (!E || !sasa) return PDGPU_ERR_ARG;
+  if (!E->lcpo) {
+    E->last_error = "LCPO component not set up";
+    return PDGPU_ERR_ARG;
+  }
+  try {
+    E->lcpo->sasa.download(sasa, E->n, E->stream);
+    E->sync();
+    return PDGPU_OK;
+  } catch (const pdgpu::EngineError &x) {
+    E->last_error = x.what();
+    return x.code;
+  }
+}

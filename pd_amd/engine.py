@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libpdgpu.so")
 PDGPU_NONBONDED = 0x1
 PDGPU_BONDED = 0x2
 PDGPU_GB = 0x4
+PDGPU_SASA = 0x8
 PDGPU_VELOCITY_VERLET = 1
 PDGPU_LANGEVIN = 3
 
@@ -31,7 +32,8 @@ EXPORTS = (
     "pdgpu_create", "pdgpu_destroy", "pdgpu_last_error", "pdgpu_version", "pdgpu_set_atom_params",
     "pdgpu_set_molecules", "pdgpu_set_bond_order", "pdgpu_set_bond_order_from_cov", "pdgpu_set_space",
     "pdgpu_request_cutoff", "pdgpu_set_padding", "pdgpu_calc_shadow", "pdgpu_setup_nonbonded",
-    "pdgpu_set_bonded", "pdgpu_setup_gb", "pdgpu_set_positions", "pdgpu_get_positions",
+    "pdgpu_set_bonded", "pdgpu_setup_gb", "pdgpu_setup_lcpo", "pdgpu_get_sasa", "pdgpu_set_positions",
+    "pdgpu_get_positions",
     "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_build_list", "pdgpu_full_update_count",
     "pdgpu_get_pair_count", "pdgpu_get_pairs", "pdgpu_compute", "pdgpu_get_born_radii", "pdgpu_md",
     "pdgpu_set_timing", "pdgpu_kernel_time",
@@ -63,6 +65,11 @@ class NBParams(C.Structure):
                 ("ForceSwitch", C.c_int32), ("EnergySwitch", C.c_int32), ("VdwCor", C.c_int32),
                 ("VdwCorDensity", C.c_double), ("VdwCorEpsilon", C.c_double),
                 ("VdwCorRadius", C.c_double), ("VdwCorCutoff", C.c_double)]
+
+
+class LCPOParams(C.Structure):
+    """FF_SASA_LCPO parameters (lcpo.h:59-68)."""
+    _fields_ = [("GlobalASP", C.c_double), ("UpdateSasa", C.c_int32)]
 
 
 class GBParams(C.Structure):
@@ -143,6 +150,8 @@ def load():
     L.pdgpu_get_pairs.argtypes = [vp, ip, C.POINTER(C.c_uint32), C.c_int64]
     L.pdgpu_compute.argtypes = [vp, C.c_uint32, dp, C.c_size_t, C.POINTER(Energies)]
     L.pdgpu_get_born_radii.argtypes = [vp, dp]
+    L.pdgpu_setup_lcpo.argtypes = [vp, C.POINTER(LCPOParams), dp, dp, C.POINTER(C.c_int8), dp]
+    L.pdgpu_get_sasa.argtypes = [vp, dp]
     L.pdgpu_md.argtypes = [vp, C.POINTER(MDParams), C.c_int32, dp, dp]
     L.pdgpu_set_timing.argtypes = [vp, C.c_int32]
     L.pdgpu_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
@@ -248,6 +257,20 @@ class Engine:
                 v = [0.0] if conv is _d else [0]
             keep.append(conv(v))
         self._chk(self.L.pdgpu_setup_gb(self.h, C.byref(p), *[q for _, q in keep]))
+
+    def setup_lcpo(self, p: LCPOParams, radius, P, use, asp=None):
+        """FF_SASA_LCPO::setup with per-atom LCPOSASATYPE radius, P1..P4 (n,4), use flags."""
+        r = _d(radius)
+        pp = _d(np.asarray(P, dtype=np.float64).reshape(-1))
+        u = np.ascontiguousarray(use, dtype=np.int8)
+        a = _d(asp) if asp is not None else (None, None)
+        self._chk(self.L.pdgpu_setup_lcpo(self.h, C.byref(p), r[1], pp[1],
+                                          u.ctypes.data_as(C.POINTER(C.c_int8)), a[1]))
+
+    def sasa(self):
+        out = np.zeros(self.n)
+        self._chk(self.L.pdgpu_get_sasa(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def set_bonded(self, s):
         """FF_Bonded terms of a MolSystem (bonds, angles, torsions)."""

@@ -51,6 +51,7 @@ class MolSystem:
     tor_gamma: np.ndarray        # (nt,4) rad
     box: Optional[np.ndarray] = None   # (3,) for a periodic box, None for vacuum
     cov: dict = field(default_factory=dict)
+    atype: list = field(default_factory=list)   # force-field atom type names (peptides)
 
     @property
     def n(self) -> int:
@@ -537,7 +538,73 @@ def peptide_chain(n_res: int, seed: int = 7, phi_mu: float = -60.0, phi_sd: floa
         tor_vn=tor_vn, tor_n=tor_n, tor_gamma=tor_gamma, box=None,
     )
     s.cov = build_covalent(n, s.bonds, s.bond_l, s.angles, s.angle_t0)
+    s.atype = list(types)
     return s
+
+
+# LCPO parameters (Weiser, Shenkin & Still 1999) as PD's amber03aa.ff lists them
+# in its LCPOSASAPARAMETERS section (param/amber03aa.ff:136-210):
+# (type, heavy-atom neighbours) -> (radius A, P1, P2, P3, P4).  Hydrogens are not
+# used (FF_SASA_LCPO::setup, lcpo.cpp:110-114).
+_SP2_2 = (1.70, 5.1245e-01, -1.5966e-01, -1.9781e-04, 1.6392e-04)
+_SP2_3 = (1.70, 7.0344e-02, -1.9015e-02, -2.2009e-05, 1.6875e-05)
+LCPO_TYPES = {
+    ("C", 3): _SP2_3, ("CA", 2): _SP2_2, ("CA", 3): _SP2_3,
+    ("CT", 1): (1.70, 7.7887e-01, -2.8063e-01, -1.2968e-03, 3.9328e-04),
+    ("CT", 2): (1.70, 5.6482e-01, -1.9608e-01, -1.0219e-03, 2.6580e-04),
+    ("CT", 3): (1.70, 2.3348e-01, -7.2627e-02, -2.0079e-04, 7.9670e-05),
+    ("CT", 4): (1.70, 0.0, 0.0, 0.0, 0.0),
+    ("N", 1): (1.65, 7.8602e-01, -2.9198e-01, -6.5370e-04, 3.6247e-04),
+    ("N", 2): (1.65, 4.1102e-01, -1.2254e-01, -7.5448e-05, 1.1804e-04),
+    ("N", 3): (1.65, 6.2577e-02, -1.7874e-02, -8.3120e-05, 1.9849e-05),
+    ("N3", 1): (1.65, 7.8602e-01, -2.9198e-01, -6.5370e-04, 3.6247e-04),
+    ("N3", 2): (1.65, 4.1102e-01, -1.2254e-01, -7.5448e-05, 1.1804e-04),
+    ("O", 1): (1.60, 6.8563e-01, -1.8680e-01, -1.3557e-03, 2.3743e-04),
+    ("OH", 1): (1.60, 7.7914e-01, -2.5262e-01, -1.6056e-03, 3.5071e-04),
+    ("O2", 1): (1.60, 8.8857e-01, -3.3421e-01, -1.8683e-03, 4.9372e-04),
+    ("S", 2): (1.90, 5.4581e-01, -1.9477e-01, -1.2873e-03, 2.9247e-04),
+    ("SH", 1): (1.90, 7.7220e-01, -2.6393e-01, 1.0629e-03, 2.1790e-04),
+}
+
+
+def lcpo_params(s: MolSystem):
+    """Per-atom LCPO inputs (use, radius, P1..P4) for a peptide MolSystem, by the
+    lookup of FF_SASA_LCPO::setup (lcpo.cpp:68-107): atom type and the number of
+    covalent neighbours that are not hydrogens.  The chains built here have sp2
+    carbons with fewer heavy neighbours than the table covers (the C-terminal
+    carbonyl carbon, the open ends of the unclosed aromatic side chains): such a
+    carbon takes the sp2 row with two neighbours, or the sp3 (CT) row of its
+    neighbour count when it has one."""
+    if not s.atype:
+        raise ValueError("LCPO parameters need atom types (peptide systems)")
+    n = s.n
+    heavy = np.zeros(n, dtype=np.int64)
+    isH = np.array([t[0] == "H" for t in s.atype])
+    for a, b in s.bonds.tolist():
+        if not isH[b]:
+            heavy[a] += 1
+        if not isH[a]:
+            heavy[b] += 1
+    use = (~isH).astype(np.int8)
+    radius = np.zeros(n)
+    P = np.zeros((n, 4))
+    for i, t in enumerate(s.atype):
+        if isH[i]:
+            continue
+        key = (t, int(heavy[i]))
+        if key in LCPO_TYPES:
+            row = LCPO_TYPES[key]
+        elif t in ("C", "CA") and key[1] == 2:
+            row = _SP2_2
+        elif t in ("C", "CA") and ("CT", key[1]) in LCPO_TYPES:
+            row = LCPO_TYPES[("CT", key[1])]
+        elif key in LCPO_TYPES:
+            row = LCPO_TYPES[key]
+        else:
+            raise ValueError(f"no LCPO type for atom {i} ({t}, {heavy[i]} heavy neighbours)")
+        radius[i] = row[0]
+        P[i] = row[1:]
+    return use, radius, P
 
 
 def min_nonbonded_distance(s: MolSystem) -> float:

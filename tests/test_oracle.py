@@ -504,3 +504,102 @@ def test_kinetic_energy():
     m = np.array([2.0, 3.0])
     v = np.array([[1.0, 2.0, 2.0], [0.0, 3.0, 4.0]])
     assert O.kinetic_energy(m, v) == pytest.approx(0.5 * 2 * 9 + 0.5 * 3 * 25, rel=1e-15)
+
+
+# ---------------------------------------------------------------- LCPO SASA
+
+def _all_pairs_list(n):
+    """PD's half list holding every pair (atom i: all j < i, order 7)."""
+    counts = np.arange(n, dtype=np.int32)
+    packed = np.concatenate([np.arange(i, dtype=np.uint32) | np.uint32(7 << 24) for i in range(n)]) \
+        if n > 1 else np.zeros(0, np.uint32)
+    return counts, packed.astype(np.uint32)
+
+
+def _lcpo(pos, rad, P, use=None, sig=None):
+    n = len(pos)
+    use = np.ones(n, np.int8) if use is None else use
+    sig = np.full(n, 1e-21) if sig is None else sig
+    c, pk = _all_pairs_list(n)
+    return O.lcpo_forces(pos, use, rad, P, sig, c, pk)
+
+
+def _lcpo_brute(pos, rad, P, use):
+    """Independent LCPO SASA: overlap partners sorted by index, first 80 kept
+    (Nlistmax), triple sum with the i-k overlap condition, clamp at zero."""
+    n = len(pos)
+    R = rad + 1.4
+    d = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=2)
+
+    def s(i, j):
+        return 2 * math.pi * R[i] * (R[i] - d[i, j] / 2 - (R[i] ** 2 - R[j] ** 2) / (2 * d[i, j]))
+
+    nb = [[j for j in range(n) if j != i and use[i] and use[j] and d[i, j] < R[i] + R[j]][:80] for i in range(n)]
+    out = np.zeros(n)
+    for i in range(n):
+        if not use[i]:
+            continue
+        s2 = s3 = s4 = 0.0
+        for j in nb[i]:
+            t3 = sum(s(j, k) for k in nb[j] if k != i and d[i, k] <= R[i] + R[k])
+            s2 += s(i, j)
+            s3 += t3
+            s4 += s(i, j) * t3
+        out[i] = max(0.0, P[i, 0] * 4 * math.pi * R[i] ** 2 + P[i, 1] * s2 + P[i, 2] * s3 + P[i, 3] * s4)
+    return out
+
+
+def test_lcpo_isolated_atom_and_pair_closed_forms():
+    P = np.array([[0.6, -0.2, -0.001, 0.0003]] * 2)
+    rad = np.array([1.7, 1.6])
+    f, sasa, e = _lcpo(np.array([[0.0, 0.0, 0.0], [50.0, 0.0, 0.0]]), rad, P)
+    assert sasa[0] == pytest.approx(0.6 * 4 * math.pi * 3.1 ** 2, rel=1e-15)
+    assert np.all(f == 0.0)
+    d = 2.9
+    f, sasa, e = _lcpo(np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]]), rad, P)
+    r1, r2 = 3.1, 3.0
+    s12 = 2 * math.pi * r1 * (r1 - d / 2 - (r1 ** 2 - r2 ** 2) / (2 * d))
+    assert sasa[0] == pytest.approx(0.6 * 4 * math.pi * r1 ** 2 - 0.2 * s12, rel=1e-14)
+    assert e == pytest.approx(1e-21 * sasa.sum(), rel=1e-14)
+    assert f[0, 0] == pytest.approx(-f[1, 0], rel=1e-12)  # pair forces act along the axis, opposite
+
+
+@pytest.mark.parametrize("term", [1, 2, 3])
+def test_lcpo_forces_are_gradients_per_term(term):
+    """P2, P3 and P4 terms separately on unclamped clusters: F = -dE/dx."""
+    pos = np.array([[0, 0, 0], [2.5, 0.3, 0.1], [1.0, 2.2, -0.4], [1.5, 1.0, 2.0], [3.1, 2.4, 1.2]], float)
+    rad = np.array([1.7, 1.6, 1.65, 1.9, 1.7])
+    P = np.zeros((5, 4))
+    P[:, 0] = 0.5
+    P[:, term] = (-0.15, -0.01, 0.0016)[term - 1]
+    f, sasa, e = _lcpo(pos, rad, P)
+    assert np.all(sasa > 0)
+    h = 1e-6
+    for i in range(5):
+        for c in range(3):
+            p1 = pos.copy(); p1[i, c] += h
+            p2 = pos.copy(); p2[i, c] -= h
+            fd = -(_lcpo(p1, rad, P)[2] - _lcpo(p2, rad, P)[2]) / (2 * h * U.ANGSTROM)
+            assert abs(f[i, c] - fd) <= 1e-6 * np.abs(f).max(), (i, c)
+
+
+def test_lcpo_matches_brute_force_with_truncation_and_clamp():
+    """A dense cluster: atoms with more than 80 overlap partners, buried atoms
+    clamped at zero, hydrogens excluded."""
+    rng = np.random.default_rng(5)
+    n = 140
+    pos = rng.uniform(-5.0, 5.0, (n, 3))
+    rad = rng.choice([1.6, 1.65, 1.7, 1.9], n)
+    use = (rng.random(n) > 0.1).astype(np.int8)
+    P = np.column_stack([rng.uniform(0.05, 0.8, n), rng.uniform(-0.3, -0.02, n),
+                         rng.uniform(-2e-3, 1e-3, n), rng.uniform(1e-5, 5e-4, n)])
+    f, sasa, e = _lcpo(pos, rad, P, use=use)
+    ref = _lcpo_brute(pos, rad, P, use)
+    R = rad + 1.4
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    counts = ((d < R[:, None] + R[None, :]) & (use[:, None] > 0) & (use[None, :] > 0)).sum(1) - 1
+    assert counts.max() > 80          # the Nlistmax truncation is exercised
+    assert (ref[use > 0] == 0.0).any() and (ref > 0).any()  # so is the clamp
+    np.testing.assert_allclose(sasa, ref, rtol=1e-11, atol=1e-9)
+    assert np.all(f[ref == 0.0] == 0.0)  # a clamped atom gets no force (lcpo.cpp:940-944)
+    assert np.all(f[use == 0] == 0.0) and np.all(sasa[use == 0] == 0.0)
