@@ -84,6 +84,13 @@ typedef struct pdgpu_gb_params {
 #define PDGPU_BONDED 0x2u    /* FF_Bonded bonds + angles + torsions */
 #define PDGPU_GB 0x4u        /* FF_GeneralizedBorn_Still::calcForces (FastMode) */
 #define PDGPU_SASA 0x8u      /* FF_SASA_LCPO::calcForces */
+/* With PDGPU_ENERGY_ONLY the selected components run calcEnergies() instead of
+ * calcForces() (ForcefieldBase::calcEnergies, forcefield.h:110-239): no forces
+ * (f_accum must be NULL); FF_NonBonded::calcEnergies_Full (nonbonded.cpp:1184-1202),
+ * FF_GeneralizedBorn_Still::calcEnergies (gbff.cpp:178-183, exact GB, no
+ * FastMode swap), FF_SASA_LCPO::calcLCPOSasaEnergies (lcpo.cpp:453-560); FF_Bonded
+ * energies are those of its force routine. */
+#define PDGPU_ENERGY_ONLY 0x100u
 
 /* Integrators of MolecularDynamics (md.h:76). */
 #define PDGPU_VELOCITY_VERLET 1

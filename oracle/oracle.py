@@ -133,6 +133,8 @@ def lib():
         L.pdo_remd_uniform.restype = C.c_double
         L.pdo_lcpo_forces.argtypes = [C.c_int, _dp, _bp, _dp, _dp, _dp, _ip, _up, _dp, _dp]
         L.pdo_lcpo_forces.restype = C.c_double
+        L.pdo_lcpo_energies.argtypes = [C.c_int, _dp, _bp, _dp, _dp, _dp, _ip, _up, _dp]
+        L.pdo_lcpo_energies.restype = C.c_double
         _lib = L
     return _lib
 
@@ -356,6 +358,16 @@ def lcpo_forces(pos, use, radius, P, sigma, counts, packed):
                               _f64(radius), _f64(P).reshape(-1), _f64(sigma), _i32(counts),
                               np.ascontiguousarray(packed, dtype=np.uint32), f, sasa)
     return f.reshape(-1, 3), sasa, e
+
+
+def lcpo_energies(pos, use, radius, P, sigma, counts, packed):
+    """FF_SASA_LCPO::calcLCPOSasaEnergies (lcpo.cpp:453-560): (per-atom SASA, energy J)."""
+    n = len(radius)
+    sasa = np.zeros(n)
+    e = lib().pdo_lcpo_energies(n, _f64(pos).reshape(-1), np.ascontiguousarray(use, dtype=np.int8),
+                                _f64(radius), _f64(P).reshape(-1), _f64(sigma), _i32(counts),
+                                np.ascontiguousarray(packed, dtype=np.uint32), sasa)
+    return sasa, e
 
 
 def decode(packed):

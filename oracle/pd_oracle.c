@@ -1312,3 +1312,81 @@ double pdo_lcpo_forces(int n, const double *pos, const int8_t *use, const double
   free(R);
   return epot;
 }
+
+/* calcLCPOSasaEnergies(true), lcpo.cpp:453-560: the energy-only LCPO of the MC
+ * path.  Differs from the force routine in its overlap test on the distance
+ * itself, sphereSurfaceOverlap (maths.cpp:120-123) clamped at zero, and the
+ * i-k test (R_i + R_k) < d_ik.  Writes per-atom SASA, returns the energy (J). */
+double pdo_lcpo_energies(int n, const double *pos, const int8_t *use, const double *radius,
+                         const double *P, const double *sigma, const int32_t *counts,
+                         const uint32_t *packed, double *sasa) {
+  const double TwoPI = 2.0 * PD_PI, FourPI = 4.0 * PD_PI;
+  int *nsz = (int *)calloc((size_t)n, sizeof(int));
+  int *nidx = (int *)malloc((size_t)n * LCPO_NMAX * sizeof(int));
+  double *ns = (double *)malloc((size_t)n * LCPO_NMAX * sizeof(double));
+  double *R = (double *)malloc((size_t)n * sizeof(double));
+  for (int i = 0; i < n; i++) R[i] = radius[i] + 1.4;
+  int64_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const int64_t beg = off;
+    off += counts[i];
+    if (!use[i]) continue;
+    for (int64_t nj = beg; nj < beg + counts[i]; nj++) {
+      const int j = (int)(packed[nj] & 0xFFFFFFu);
+      if (j >= i) break;
+      if (!use[j]) continue;
+      const double g = pos[3 * i] - pos[3 * j], h = pos[3 * i + 1] - pos[3 * j + 1],
+                   k = pos[3 * i + 2] - pos[3 * j + 2];
+      const double dij = sqrt(g * g + h * h + k * k);
+      if (dij < (R[i] + R[j])) {
+        if (nsz[i] < LCPO_NMAX) {
+          double s = TwoPI * R[i] * (R[i] - dij / 2.0 - (R[i] * R[i] - R[j] * R[j]) / (2.0 * dij));
+          nidx[i * LCPO_NMAX + nsz[i]] = j;
+          ns[i * LCPO_NMAX + nsz[i]] = s < 0 ? 0 : s;
+          nsz[i]++;
+        }
+        if (nsz[j] < LCPO_NMAX) {
+          double s = TwoPI * R[j] * (R[j] - dij / 2.0 - (R[j] * R[j] - R[i] * R[i]) / (2.0 * dij));
+          nidx[j * LCPO_NMAX + nsz[j]] = i;
+          ns[j * LCPO_NMAX + nsz[j]] = s < 0 ? 0 : s;
+          nsz[j]++;
+        }
+      }
+    }
+  }
+  double epot = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (!use[i]) {
+      sasa[i] = 0;
+      continue;
+    }
+    const double Si = FourPI * R[i] * R[i];
+    double Si2 = 0, Si3 = 0, Si4 = 0;
+    for (int jn = 0; jn < nsz[i]; jn++) {
+      const int ja = nidx[i * LCPO_NMAX + jn];
+      const double Si2_t = ns[i * LCPO_NMAX + jn];
+      double Si3_t = 0;
+      for (int kn = 0; kn < nsz[ja]; kn++) {
+        const int ka = nidx[ja * LCPO_NMAX + kn];
+        if (ka == i) continue;
+        const double g = pos[3 * ka] - pos[3 * i], h = pos[3 * ka + 1] - pos[3 * i + 1],
+                     k = pos[3 * ka + 2] - pos[3 * i + 2];
+        if ((R[i] + R[ka]) < sqrt(g * g + h * h + k * k)) continue;
+        Si3_t += ns[ja * LCPO_NMAX + kn];
+      }
+      Si4 += (Si3_t * Si2_t);
+      Si3 += Si3_t;
+      Si2 += Si2_t;
+    }
+    const double *Pi = P + 4 * i;
+    double SAi = Pi[0] * Si + Pi[1] * Si2 + Pi[2] * Si3 + Pi[3] * Si4;
+    if (SAi < 0) SAi = 0;
+    sasa[i] = SAi;
+    epot += SAi * sigma[i];
+  }
+  free(nsz);
+  free(nidx);
+  free(ns);
+  free(R);
+  return epot;
+}

@@ -205,6 +205,12 @@ double pdo_lcpo_forces(int n, const double *pos, const int8_t *use, const double
                        const double *P, const double *sigma, const int32_t *counts,
                        const uint32_t *packed, double *f, double *sasa);
 
+/* FF_SASA_LCPO::calcLCPOSasaEnergies(true) (lcpo.cpp:453-560), the energy-only
+ * variant: writes per-atom SASA, returns the cavity energy (J). */
+double pdo_lcpo_energies(int n, const double *pos, const int8_t *use, const double *radius,
+                         const double *P, const double *sigma, const int32_t *counts,
+                         const uint32_t *packed, double *sasa);
+
 #ifdef __cplusplus
 }
 #endif

@@ -436,6 +436,7 @@ int pdgpu_compute(pdgpu_engine *h, uint32_t components, double *f_accum, size_t 
     E.require(E.have_positions, "positions not set");
     E.require(components != 0, "no component selected");
     E.require(!f_accum || stride_bytes >= 3 * sizeof(double), "bad force stride");
+    E.require(!(f_accum && (components & PDGPU_ENERGY_ONLY)), "PDGPU_ENERGY_ONLY computes no forces: f_accum must be NULL");
     E.evaluate(components);
     if (f_accum) E.forces_to_host(f_accum, stride_bytes);
     pdgpu_energies tmp;

@@ -136,6 +136,7 @@ struct NBKernelParams {
 // Kernel constants of FF_GeneralizedBorn_Still in kcal/mol and A.
 struct GBK {
   double cut2;          // Cutoff^2
+  double cut;           // Cutoff
   double still2;        // GbsaStillCutoff^2
   double vdwcut2;       // VdwCutoff^2
   float premul;         // -halfeconv (1/eps_in - 1/eps_out), kcal/mol A
@@ -197,7 +198,7 @@ struct GBState {
                   const std::vector<int> &spj, const std::vector<int> &spo);
   void upload(hipStream_t s);
   void run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
-           long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner);
+           long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner, bool energy_only = false);
 };
 
 // ---- LCPO SASA (k_lcpo.hip) ------------------------------------------------------
@@ -219,7 +220,7 @@ struct LCPOState {
   // full: recompute now; otherwise recompute when step % update == 0 (step from
   // step_ptr on the device when given, else step_host) and re-apply otherwise
   void run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
-           long long *fz, const int *step_ptr, int step_host, bool full);
+           long long *fz, const int *step_ptr, int step_host, bool full, bool energy_only = false);
 };
 
 void gb_fixed_terms(int n, const double *gbrad, const pdgpu_gb_params &p, const int32_t *off12,
@@ -328,7 +329,7 @@ struct Engine {
   void prepare_slots();
   void setup_nonbonded(const pdgpu_nb_params &p);
   void gather_positions();
-  void launch_nonbonded(double *energy_slots);  // energies: vdw, elec, virial
+  void launch_nonbonded(double *energy_slots, bool forces = true);  // energies: vdw, elec, virial
   // bonded.hip
   void launch_bonded(double *energy_slots);      // energies: bond, angle, torsion, virial
   // integrate.hip
@@ -342,11 +343,11 @@ struct Engine {
   // k_lcpo.hip
   std::unique_ptr<LCPOState> lcpo;
   DBuf<double> e_partial_lcpo;
-  void launch_lcpo(double *energy_slot, int step, bool full);
+  void launch_lcpo(double *energy_slot, int step, bool full, bool energy_only = false);
   // k_gb.hip
   bool have_gb = false;
   std::unique_ptr<GBState> gb;
-  void launch_gb(double *energy_slots);
+  void launch_gb(double *energy_slots, bool energy_only = false);
   void gb_energies_to_host(const double *slots, pdgpu_energies &e);
 };
 

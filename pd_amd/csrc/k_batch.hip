@@ -294,6 +294,8 @@ struct Batch {
   // protocol run of p.total_steps steps.
   void md(const pdgpu_md_params &p, int n, double *ep_trace, double *ek_trace) {
     if (n <= 0) return;
+    if (p.components & PDGPU_ENERGY_ONLY)
+      throw EngineError(PDGPU_ERR_ARG, "MD needs forces: PDGPU_ENERGY_ONLY is for pdgpu_batch_compute");
     BatchLangevin L{};
     L.t = p.timestep;
     L.gamma = p.fric_coeff;
@@ -612,10 +614,12 @@ int pdgpu_batch_compute(pdgpu_batch *h, uint32_t components, double *f_accum, pd
     b.zero_energies();
     hipLaunchKernelGGL(pdgpu::k_zero3n, dim3((b.natoms + T - 1) / T), dim3(T), 0, b.stream, b.fx.p, b.fy.p, b.fz.p,
                        b.natoms);
+    const bool eonly = (components & PDGPU_ENERGY_ONLY) != 0;
+    if (eonly && f_accum) throw EngineError(PDGPU_ERR_ARG, "PDGPU_ENERGY_ONLY computes no forces: f_accum must be NULL");
     if ((components & PDGPU_GB) && b.have_gb)
-      b.gb->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.e_sys.p, pdgpu::kStride, 0, nullptr);
+      b.gb->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.e_sys.p, pdgpu::kStride, 0, nullptr, eonly);
     if ((components & PDGPU_SASA) && b.lcpo) {
-      b.lcpo->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, nullptr, 0, true);
+      b.lcpo->run(b.stream, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, nullptr, 0, true, eonly);
       hipLaunchKernelGGL(pdgpu::k_seg_sum, dim3(b.nsys), dim3(256), 0, b.stream, b.lcpo->eatom.p, b.atom_beg.p,
                          b.atom_end.p, b.e_sys.p, pdgpu::kStride, 3);
     }

@@ -134,6 +134,11 @@ __global__ void k_gb_finalize_radii(int n, double halfeconv, double offset, doub
 }
 
 // ---- pass 2: GB cross terms, vacuum Coulomb and LJ ----------------------------------
+// EONLY: FF_GeneralizedBorn_Still::calcEnergies instead (gbff.cpp:178-183, 537-596):
+// the exact GB cross term on every pair within Cutoff, whatever its bond order and
+// with no ExpApproxThreshold swap, plus FF_NonBonded::calcEnergies with
+// Dielectric = DielectricSolute and energy switching; no forces, no dE/da.
+template <bool EONLY>
 __global__ __launch_bounds__(64) void k_gb_pair(
     GBK K, const GBWork *__restrict__ work, const GBSys *__restrict__ sys, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const float *__restrict__ q,
@@ -181,12 +186,18 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       dz = zi - sz[jj];
       r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1588-1591
     }
-    on = on && r2 <= K.cut2;
+    bool on_gb = false;
+    if (EONLY) {
+      on_gb = on && sqrt(r2) <= K.cut;  // calcBornEnergy's test on the distance (gbff.cpp:560)
+      on = on && r2 <= K.cut2;
+    } else {
+      on = on && r2 <= K.cut2;
+    }
     int bo = 7;
     if (w.special && on)
       bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off, sp_off + S.sp_base, sp_part,
                          sp_ord);
-    const float r2f = on ? (float)r2 : 1.0f;
+    const float r2f = (on || on_gb) ? (float)r2 : 1.0f;
     const float rinv = __builtin_amdgcn_rsqf(r2f);
     const float r = r2f * rinv;
     const float aj = sa[jj];
@@ -201,7 +212,10 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const float dS = -K.invw2_4 * st * S0;
     const float Sw = S0 * S0;
     float ep = 0.0f, dedr = 0.0f, dal = 0.0f, Eel = 0.0f, Fel = 0.0f;
-    if (uGB > K.thresh) {
+    if (EONLY) {
+      ep = on_gb ? K.premul * qq / sqrtf(r2f + aiaj * __expf(-r2f * 0.25f / aiaj)) * Sw : 0.0f;
+      if (nb) Eel = K.ec_in * s14e * qq * rinv * Sw;
+    } else if (uGB > K.thresh) {
       const float tGB = __expf(uGB);
       const float lGB = 1.0f / (r2f + tGB * aiaj);
       const float fGB = sqrtf(lGB);
@@ -239,29 +253,31 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       Fv = VS * Fv + Ev * vdS;
       Ev *= VS;
     }
-    // gradient along d = p_i - p_j: GB part 2 dedr / r, vdW + elec (dE/dr) / r
-    const float g = on ? (2.0f * dedr + (nb ? (Fv + Fel) : 0.0f)) * rinv : 0.0f;
-    const float gx = g * (float)dx, gy = g * (float)dy, gz = g * (float)dz;
-    fix -= gx;
-    fiy -= gy;
-    fiz -= gz;
-    ax[jj] += gx;
-    ay[jj] += gy;
-    az[jj] += gz;
-    dai += on ? dal * aj : 0.0f;
-    ad[jj] += on ? dal * ai : 0.0f;
-    epol += on ? (double)ep : 0.0;
+    if (!EONLY) {
+      // gradient along d = p_i - p_j: GB part 2 dedr / r, vdW + elec (dE/dr) / r
+      const float g = on ? (2.0f * dedr + (nb ? (Fv + Fel) : 0.0f)) * rinv : 0.0f;
+      const float gx = g * (float)dx, gy = g * (float)dy, gz = g * (float)dz;
+      fix -= gx;
+      fiy -= gy;
+      fiz -= gz;
+      ax[jj] += gx;
+      ay[jj] += gy;
+      az[jj] += gz;
+      dai += on ? dal * aj : 0.0f;
+      ad[jj] += on ? dal * ai : 0.0f;
+    }
+    epol += (EONLY ? on_gb : on) ? (double)ep : 0.0;
     evdw += on && nb ? (double)Ev : 0.0;
     eel += on && nb ? (double)Eel : 0.0;
   }
   __syncthreads();
-  if (iv) {
+  if (!EONLY && iv) {
     fix_add(&fx[i], fix_from_float(fix));
     fix_add(&fy[i], fix_from_float(fiy));
     fix_add(&fz[i], fix_from_float(fiz));
     fix_add(&deda[i], fix_from_float(dai));
   }
-  if (lane < nj) {
+  if (!EONLY && lane < nj) {
     const int j = w.j0 + lane;
     fix_add(&fx[j], fix_from_float(ax[lane]));
     fix_add(&fy[j], fix_from_float(ay[lane]));
@@ -465,6 +481,7 @@ void GBState::set_params(const pdgpu_gb_params &p) {
   const double ec = Consts::econv();
   GBK k{};
   k.cut2 = p.Cutoff * p.Cutoff;
+  k.cut = p.Cutoff;
   k.still2 = p.GbsaStillCutoff * p.GbsaStillCutoff;
   k.vdwcut2 = p.VdwCutoff * p.VdwCutoff;
   k.premul = (float)(-Consts::halfeconv() * (1 / p.DielectricSolute - 1 / p.DielectricSolvent));
@@ -571,7 +588,7 @@ void GBState::upload(hipStream_t s) {
 // per-system energies (kcal/mol) go to out[s*stride + off + {0: pol cross, 1: vdw,
 // 2: elec, 3: -}] and out[s*stride + off + 4] (pol self).
 void GBState::run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
-                  long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner) {
+                  long long *fz, double *out, int out_stride, int out_off, Engine *timer_owner, bool energy_only) {
   const int n = n_atoms_total;
   const int nw = (int)h_work.size();
   const int nsys = (int)h_sys.size();
@@ -589,14 +606,21 @@ void GBState::run(hipStream_t s, const double *x, const double *y, const double 
                      eself.p);
   {
     ScopedTimerOpt t(timer_owner, "gb_pair");
-    hipLaunchKernelGGL(k_gb_pair, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, q.p, ljr.p, lje.p, bornf.p, band.p,
-                       sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz, deda.p, e_part.p);
+    if (energy_only) {
+      hipLaunchKernelGGL(k_gb_pair<true>, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, q.p, ljr.p, lje.p, bornf.p,
+                         band.p, sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz, deda.p, e_part.p);
+    } else {
+      hipLaunchKernelGGL(k_gb_pair<false>, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, q.p, ljr.p, lje.p, bornf.p,
+                         band.p, sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz, deda.p, e_part.p);
+    }
   }
+  if (!energy_only) {
   hipLaunchKernelGGL(k_gb_chain_weights, dim3((n + 255) / 256), dim3(256), 0, s, n, kp.P4, born.p, deda.p, wgt.p);
   {
     ScopedTimerOpt t(timer_owner, "gb_chain");
     hipLaunchKernelGGL(k_gb_chain, dim3(nw), dim3(64), 0, s, K, W, SY, x, y, z, gbrad.p, Vj.p, wgt.p, band.p,
                        sp_off.p, sp_part.p, sp_ord.p, fx, fy, fz);
+  }
   }
   PDGPU_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_seg_reduce4, dim3(nsys), dim3(256), 0, s, e_part.p, work_beg.p, work_end.p, out, out_stride,
@@ -607,9 +631,9 @@ void GBState::run(hipStream_t s, const double *x, const double *y, const double 
 
 // ---- single-system glue ------------------------------------------------------------
 
-void Engine::launch_gb(double *energy_slots) {
+void Engine::launch_gb(double *energy_slots, bool energy_only) {
   require(have_gb, "GB component not set up");
-  gb->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, energy_slots, 8, 0, this);
+  gb->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, energy_slots, 8, 0, this, energy_only);
 }
 
 void Engine::gb_energies_to_host(const double *h, pdgpu_energies &e) {

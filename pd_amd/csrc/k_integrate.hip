@@ -248,16 +248,17 @@ void Engine::zero_forces() {
 }
 
 void Engine::evaluate(uint32_t components, int step) {
+  const bool eonly = (components & PDGPU_ENERGY_ONLY) != 0;
   e_out.resize(24);
   PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 24 * sizeof(double), stream));
   zero_forces();
   if (components & PDGPU_NONBONDED) {
     if (!list_valid) build_list();
-    launch_nonbonded(e_out.p + 0);
+    launch_nonbonded(e_out.p + 0, !eonly);
   }
   if ((components & PDGPU_BONDED) && (nbonds + nangles + ntors) > 0) launch_bonded(e_out.p + 4);
-  if (components & PDGPU_GB) launch_gb(e_out.p + 8);
-  if (components & PDGPU_SASA) launch_lcpo(e_out.p + 13, step, step < 0);
+  if (components & PDGPU_GB) launch_gb(e_out.p + 8, eonly);
+  if (components & PDGPU_SASA) launch_lcpo(e_out.p + 13, step, step < 0, eonly);
   last_components = components;
 }
 
@@ -398,6 +399,7 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
   require(have_positions, "positions not set");
   require(vx.n >= (size_t)n && mass.n >= (size_t)n, "velocities or masses not set");
   require(p.timestep > 0, "timestep must be positive");
+  require((p.components & PDGPU_ENERGY_ONLY) == 0, "MD needs forces: PDGPU_ENERGY_ONLY is for pdgpu_compute");
   trace.resize((size_t)std::max(n_steps, 1) * 2);
   const uint32_t comp = p.components;
   const double cor = (comp & PDGPU_NONBONDED) ? nbk.vdwcor_per_atom * n : 0.0;

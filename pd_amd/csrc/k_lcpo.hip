@@ -39,6 +39,9 @@ __device__ __forceinline__ bool lcpo_full(const int *step_ptr, int step_host, in
   return (step % update) == 0;
 }
 
+// EONLY: the overlap lists of calcLCPOSasaEnergies (lcpo.cpp:469-520): the test
+// on the distance itself and sphereSurfaceOverlap clamped at zero.
+template <bool EONLY>
 __global__ __launch_bounds__(64) void k_lcpo_lists(int n, const int *__restrict__ atom_sys,
                                                    const int *__restrict__ sys_first,
                                                    const int *__restrict__ sys_count, const double *__restrict__ x,
@@ -68,22 +71,32 @@ __global__ __launch_bounds__(64) void k_lcpo_lists(int n, const int *__restrict_
       const double g = xi - x[j], h = yi - y[j], k = zi - z[j];
       d2 = g * g + h * h + k * k;
       rj = R[j];
-      ok = d2 < (ri + rj) * (ri + rj);
+      ok = EONLY ? sqrt(d2) < (ri + rj) : d2 < (ri + rj) * (ri + rj);
     }
     const unsigned long long b = __ballot(ok);
     const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
     if (ok && pos < kLcpoMax) {
       const double d = sqrt(d2);
-      const double inv = 1 / d;
-      const double t = 0.5 * (ri * ri - rj * rj) * inv;
+      double sij;
+      if (EONLY) {  // sphereSurfaceOverlap (maths.cpp:120-123), negative overlaps set to 0
+        sij = TwoPI * ri * (ri - d / 2.0 - (ri * ri - rj * rj) / (2.0 * d));
+        if (sij < 0) sij = 0;
+      } else {
+        const double inv = 1 / d;
+        const double t = 0.5 * (ri * ri - rj * rj) * inv;
+        sij = TwoPI * ri * (ri - d * 0.5 - t);
+      }
       nl_idx[(size_t)i * kLcpoMax + pos] = j;
-      nl_s[(size_t)i * kLcpoMax + pos] = TwoPI * ri * (ri - d * 0.5 - t);
+      nl_s[(size_t)i * kLcpoMax + pos] = sij;
     }
     cnt += __popcll(b);
   }
   if (lane == 0) nl_cnt[i] = min(cnt, kLcpoMax);
 }
 
+// EONLY: calcLCPOSasaEnergies (lcpo.cpp:522-556): the SASA alone, k excluded
+// when (R_i + R_k) < d_ik.
+template <bool EONLY>
 __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                   const double *__restrict__ z, const int8_t *__restrict__ use,
                                                   const double *__restrict__ R, const double4 *__restrict__ P,
@@ -99,7 +112,7 @@ __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restric
   double der[3] = {0.0, 0.0, 0.0};
   if (!use[i]) {
     sasa[i] = 0.0;
-    dvx[i] = dvy[i] = dvz[i] = 0.0;
+    if (!EONLY) dvx[i] = dvy[i] = dvz[i] = 0.0;
     return;
   }
   const double pi[3] = {x[i], y[i], z[i]};
@@ -134,6 +147,11 @@ __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restric
       const double rk = R[ka];
       const double hx = pk[0] - pi[0], hy = pk[1] - pi[1], hz = pk[2] - pi[2];
       double dik = hx * hx + hy * hy + hz * hz;
+      if (EONLY) {
+        if ((ri + rk) < sqrt(dik)) continue;
+        Si3_t += nl_s[(size_t)ja * kLcpoMax + kn];
+        continue;
+      }
       if ((ri + rk) * (ri + rk) < dik) continue;
       dik = sqrt(dik);
       const double invik = 1 / dik;
@@ -182,6 +200,7 @@ __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restric
     der[0] = der[1] = der[2] = 0.0;
   }
   sasa[i] = SAi;
+  if (EONLY) return;  // the stored derivatives of the last force evaluation stay
   dvx[i] = der[0];
   dvy[i] = der[1];
   dvz[i] = der[2];
@@ -202,6 +221,12 @@ __global__ void k_lcpo_apply(int n, double to_kcal, const double *__restrict__ s
     fix_add(&fy[i], fix_from_double(gy * to_kcal));
     fix_add(&fz[i], fix_from_double(gz * to_kcal));
   }
+}
+
+__global__ void k_lcpo_energy(int n, double to_kcal, const double *__restrict__ sasa, const double *__restrict__ sig,
+                              double *__restrict__ eatom) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) eatom[i] = sasa[i] * sig[i] * to_kcal;
 }
 
 }  // namespace
@@ -258,14 +283,23 @@ void LCPOState::upload(hipStream_t s) {
 }
 
 void LCPOState::run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
-                    long long *fz, const int *step_ptr, int step_host, bool full) {
-  const int upd = full ? 0 : update;
-  hipLaunchKernelGGL(k_lcpo_lists, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z, use.p,
-                     R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
-  hipLaunchKernelGGL(k_lcpo_sasa, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p, step_ptr,
-                     step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
-  hipLaunchKernelGGL(k_lcpo_apply, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p, sig.p,
-                     dvx.p, dvy.p, dvz.p, fx, fy, fz, eatom.p);
+                    long long *fz, const int *step_ptr, int step_host, bool full, bool energy_only) {
+  const int upd = (full || energy_only) ? 0 : update;
+  if (energy_only) {
+    hipLaunchKernelGGL(k_lcpo_lists<true>, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z,
+                       use.p, R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
+    hipLaunchKernelGGL(k_lcpo_sasa<true>, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p,
+                       step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
+    hipLaunchKernelGGL(k_lcpo_energy, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
+                       sig.p, eatom.p);
+  } else {
+    hipLaunchKernelGGL(k_lcpo_lists<false>, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z,
+                       use.p, R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
+    hipLaunchKernelGGL(k_lcpo_sasa<false>, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p,
+                       step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
+    hipLaunchKernelGGL(k_lcpo_apply, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
+                       sig.p, dvx.p, dvy.p, dvz.p, fx, fy, fz, eatom.p);
+  }
   PDGPU_HIP(hipGetLastError());
 }
 
@@ -290,11 +324,11 @@ __global__ void k_block_sum(int n, const double *__restrict__ v, double *__restr
 }
 }  // namespace
 
-void Engine::launch_lcpo(double *energy_slot, int step, bool full) {
+void Engine::launch_lcpo(double *energy_slot, int step, bool full, bool energy_only) {
   require(lcpo != nullptr, "LCPO component not set up");
   {
     ScopedTimer t(*this, "lcpo");
-    lcpo->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, nullptr, step, full);
+    lcpo->run(stream, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, nullptr, step, full, energy_only);
   }
   const int blocks = (n + 255) / 256;
   e_partial_lcpo.resize((size_t)blocks * 4);

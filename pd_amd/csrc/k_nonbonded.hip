@@ -154,7 +154,7 @@ __device__ __forceinline__ void lds_fix_add(long long *p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
 }
 
-template <int ELEC, bool VDW>
+template <int ELEC, bool VDW, bool FORCES>
 __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_nonbonded(
     NBKernelParams P, const int *__restrict__ n_groups, const int *__restrict__ grp_first,
     const int *__restrict__ grp_count, const int *__restrict__ wg_ncand, const int *__restrict__ wg_cand,
@@ -300,14 +300,16 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
             dEdr += outer ? d_out : d_in;
           }
         }
-        const float sc = on ? dEdr * rinv : 0.0f;
-        const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
-        fix[a] += gx;
-        fiy[a] += gy;
-        fiz[a] += gz;
-        fjx -= gx;
-        fjy -= gy;
-        fjz -= gz;
+        if (FORCES) {
+          const float sc = on ? dEdr * rinv : 0.0f;
+          const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
+          fix[a] += gx;
+          fiy[a] += gy;
+          fiz[a] += gz;
+          fjx -= gx;
+          fjy -= gy;
+          fjz -= gz;
+        }
         ev8 += on ? Ev : 0.0f;
         ee8 += on ? Ee : 0.0f;
         vir8 += on ? r * dEdr : 0.0f;
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
       ev += (double)ev8;
       ee += (double)ee8;
       vir += (double)vir8;
-      if (lm) {
+      if (FORCES && lm) {
         const long long vx = fix_from_float(fjx), vy = fix_from_float(fjy), vz = fix_from_float(fjz);
         if (rank < (unsigned)kGroupRanks) {
           const int l = (int)rank * kClusterSize + (j & (kClusterSize - 1));
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
     }
     // i forces: the cluster is rank `wave` of its own group's table
 #pragma unroll
-    for (int a = 0; a < 8; a++) {
+    for (int a = 0; a < 8 && FORCES; a++) {
       const float gx = wave_sum_f(fix[a]);
       const float gy = wave_sum_f(fiy[a]);
       const float gz = wave_sum_f(fiz[a]);
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
   __syncthreads();
   // flush the touched slots of the candidate table, consecutive threads on
   // consecutive slots of a j-cluster
-  const int ns = nr * kClusterSize;
+  const int ns = FORCES ? nr * kClusterSize : 0;
   for (int t = threadIdx.x; t < 3 * ns; t += blockDim.x) {
     const int c = t / ns, l = t - c * ns;
     const long long v = jacc[c][l];
@@ -366,13 +368,21 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
   }
 }
 
-template <int ELEC, bool VDW>
+template <int ELEC, bool VDW, bool FORCES>
 void launch(Engine &E) {
-  hipLaunchKernelGGL((k_nonbonded<ELEC, VDW>), dim3(E.max_groups), dim3(64 * kGroup), 0, E.stream, E.nbk,
+  hipLaunchKernelGGL((k_nonbonded<ELEC, VDW, FORCES>), dim3(E.max_groups), dim3(64 * kGroup), 0, E.stream, E.nbk,
                      E.n_groups_d.p, E.grp_first.p, E.grp_count.p, E.wg_ncand.p, E.wg_cand.p,
                      E.periodic ? E.box[0] : 0.0, E.periodic ? E.box[1] : 0.0, E.periodic ? E.box[2] : 0.0,
                      E.spos.p, E.sprm.p, E.pentries.p, (int)E.entry_capacity,
                      E.pcount.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
+}
+
+template <int ELEC>
+void launch_mode(Engine &E, bool vdw, bool forces) {
+  if (forces)
+    vdw ? launch<ELEC, true, true>(E) : launch<ELEC, false, true>(E);
+  else  // FF_NonBonded::calcEnergies_Full: the same kernel without forces (nonbonded.cpp:1184-1202)
+    vdw ? launch<ELEC, true, false>(E) : launch<ELEC, false, false>(E);
 }
 
 }  // namespace
@@ -482,7 +492,7 @@ void Engine::prepare_slots() {
                        n_clusters_d.p, slot_orig.p, q_scaled.p, lj_r.p, lj_e.p, sprm.p);
 }
 
-void Engine::launch_nonbonded(double *energy_out) {
+void Engine::launch_nonbonded(double *energy_out, bool forces) {
   require(have_nb, "nonbonded component not set up");
   require(list_valid, "neighbour list not built");
   const int nslots = max_clusters * kClusterSize;
@@ -507,10 +517,10 @@ void Engine::launch_nonbonded(double *energy_out) {
   {
     ScopedTimer t(*this, "nonbonded");
     switch (nbk.elec_mode) {
-      case 0: v ? launch<EL_NONE, true>(*this) : launch<EL_NONE, false>(*this); break;
-      case 1: v ? launch<EL_PLAIN, true>(*this) : launch<EL_PLAIN, false>(*this); break;
-      case 3: v ? launch<EL_ESWITCH, true>(*this) : launch<EL_ESWITCH, false>(*this); break;
-      default: v ? launch<EL_FSWITCH, true>(*this) : launch<EL_FSWITCH, false>(*this); break;
+      case 0: launch_mode<EL_NONE>(*this, v, forces); break;
+      case 1: launch_mode<EL_PLAIN>(*this, v, forces); break;
+      case 3: launch_mode<EL_ESWITCH>(*this, v, forces); break;
+      default: launch_mode<EL_FSWITCH>(*this, v, forces); break;
     }
   }
   PDGPU_HIP(hipGetLastError());

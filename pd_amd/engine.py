@@ -19,6 +19,7 @@ PDGPU_NONBONDED = 0x1
 PDGPU_BONDED = 0x2
 PDGPU_GB = 0x4
 PDGPU_SASA = 0x8
+PDGPU_ENERGY_ONLY = 0x100
 PDGPU_VELOCITY_VERLET = 1
 PDGPU_LANGEVIN = 3
 

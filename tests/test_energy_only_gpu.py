@@ -1,0 +1,94 @@
+"""The calcEnergies() path (PDGPU_ENERGY_ONLY) against the oracle.
+
+FF_NonBonded::calcEnergies_Full runs the force kernel's energy arithmetic
+without forces; FF_GeneralizedBorn_Still::calcEnergies is the exact GB energy
+(no FastMode swap, every bond order in the cross term) plus the nonbonded
+energies at DielectricSolute; FF_SASA_LCPO::calcLCPOSasaEnergies clamps
+negative overlaps.  Bars as for the force paths: 1e-5 relative per energy
+term (plus the FP32 summation allowance on cancelling Coulomb sums) for FP32
+kernels, 1e-12 for the FP64 LCPO.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pd_amd import engine as G
+from pd_amd import systems as S
+from pd_amd import units as U
+from tests.test_gb_gpu import ALL_PAIRS, SWITCHED, coulomb_abs_sum
+
+pytestmark = pytest.mark.gpu
+
+
+def test_nonbonded_energy_only_equals_force_path():
+    w = S.water_box(729, seed=3)
+    nb = G.nb_params(Cutoff=10.0, InnerCutoff=8.0, VdwCutoff=10.0, VdwInnerCutoff=8.0, ForceSwitch=1)
+    e = G.engine_for(w, nb, padding=2.0)
+    _, ef = e.compute(G.PDGPU_NONBONDED)
+    _, ee = e.compute(G.PDGPU_NONBONDED | G.PDGPU_ENERGY_ONLY, forces=False)
+    for k in ("epot", "epot_vdw", "epot_elec", "virial"):
+        assert ee[k] == pytest.approx(ef[k], rel=1e-6, abs=1e-30), k
+
+
+@pytest.mark.parametrize("kw", [ALL_PAIRS, SWITCHED], ids=["all-pairs", "switched"])
+def test_gb_energy_only_matches_oracle(kw):
+    s = S.peptide_chain(40, seed=6)
+    p = O.gb_params(**kw)
+    bo = O.bond_order(s)
+    counts, packed = O.build_list(s, s.pos, max(p.Cutoff, p.VdwCutoff), 0.5, True, bo)
+    Vj, t123, _ = O.gb_fixed_terms(s, p)
+    born = O.gb_radii(s, s.pos, Vj, t123, p, counts, packed)
+    ref = O.gb_energies(s, s.pos, born, p, counts, packed)
+    e = G.engine_for(s, None)
+    e.setup_gb(G.gb_params(**kw), s.gbrad, s.cov)
+    _, en = e.compute(G.PDGPU_GB | G.PDGPU_ENERGY_ONLY, forces=False)
+    _, fast = e.compute(G.PDGPU_GB)
+    scale = coulomb_abs_sum(s, kw)
+    for k in ("epot", "epot_pol", "epot_pol_self", "epot_pol_cross", "epot_vdw", "epot_elec"):
+        print(f"{k:15s} gpu {en[k]:.9e} ref {ref[k]:.9e} fastmode {fast[k]:.9e}")
+        allowance = 1e-6 * scale if k in ("epot_elec", "epot") else 0.0
+        assert abs(en[k] - ref[k]) <= 1e-5 * abs(ref[k]) + allowance, k
+
+
+def test_lcpo_energy_only_matches_oracle():
+    s = S.peptide_chain(25, seed=9)
+    use, rad, P = S.lcpo_params(s)
+    e = G.engine_for(s)
+    e.setup_lcpo(G.LCPOParams(GlobalASP=0.005, UpdateSasa=5), rad, P, use)
+    _, en = e.compute(G.PDGPU_SASA | G.PDGPU_ENERGY_ONLY, forces=False)
+    sig = np.full(s.n, 0.005 / U.NA / U.J2KCAL)
+    counts = np.arange(s.n, dtype=np.int32)
+    packed = np.concatenate([np.arange(i, dtype=np.uint32) | np.uint32(7 << 24) for i in range(s.n)])
+    sasa_ref, e_ref = O.lcpo_energies(s.pos, use, rad, P, sig, counts, packed.astype(np.uint32))
+    np.testing.assert_allclose(e.sasa(), sasa_ref, rtol=1e-12, atol=1e-12)
+    assert en["epot_surf"] == pytest.approx(e_ref, rel=1e-12)
+
+
+def test_energy_only_rejects_forces_and_md():
+    s = S.peptide_chain(5, seed=1)
+    e = G.engine_for(s, bonded=True)
+    with pytest.raises(G.PdGpuError):
+        e.compute(G.PDGPU_BONDED | G.PDGPU_ENERGY_ONLY, forces=True)
+    md = G.MDParams(integrator=G.PDGPU_VELOCITY_VERLET, components=G.PDGPU_BONDED | G.PDGPU_ENERGY_ONLY,
+                    timestep=1e-15, temperature=300.0, fric_coeff=0.0, update_nlist=0, step_offset=0,
+                    total_steps=10, seed=1, noise_stream=0)
+    with pytest.raises(G.PdGpuError):
+        e.md(md, 1)
+
+
+def test_batch_energy_only_equals_single():
+    sys_ = [S.peptide_chain(k, seed=30 + k) for k in (8, 14)]
+    engines = []
+    for s in sys_:
+        use, rad, P = S.lcpo_params(s)
+        e = G.engine_for(s, bonded=True)
+        e.setup_gb(G.gb_params(**ALL_PAIRS), s.gbrad, s.cov)
+        e.setup_lcpo(G.LCPOParams(GlobalASP=0.005, UpdateSasa=5), rad, P, use)
+        engines.append(e)
+    b = G.Batch(engines)
+    comps = G.PDGPU_GB | G.PDGPU_BONDED | G.PDGPU_SASA | G.PDGPU_ENERGY_ONLY
+    _, eb = b.compute(comps, forces=False)
+    for k, e in enumerate(engines):
+        _, en = e.compute(comps, forces=False)
+        for key in ("epot", "epot_pol", "epot_vdw", "epot_elec", "epot_surf", "epot_bond"):
+            assert eb[k][key] == pytest.approx(en[key], rel=1e-12, abs=1e-30), (k, key)

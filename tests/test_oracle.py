@@ -603,3 +603,26 @@ def test_lcpo_matches_brute_force_with_truncation_and_clamp():
     np.testing.assert_allclose(sasa, ref, rtol=1e-11, atol=1e-9)
     assert np.all(f[ref == 0.0] == 0.0)  # a clamped atom gets no force (lcpo.cpp:940-944)
     assert np.all(f[use == 0] == 0.0) and np.all(sasa[use == 0] == 0.0)
+
+
+def test_lcpo_energy_routine_clamps_negative_overlaps():
+    """calcLCPOSasaEnergies equals the force routine's SASA where no overlap is
+    negative, and clamps a negative overlap (nearly coincident spheres of
+    different size) at zero where the force routine keeps it."""
+    s = S.peptide_chain(10, seed=2)
+    use, rad, P = S.lcpo_params(s)
+    sig = np.full(s.n, 1e-21)
+    c, pk = _all_pairs_list(s.n)
+    sasa_e, e_e = O.lcpo_energies(s.pos, use, rad, P, sig, c, pk)
+    _, sasa_f, e_f = O.lcpo_forces(s.pos, use, rad, P, sig, c, pk)
+    np.testing.assert_allclose(sasa_e, sasa_f, rtol=1e-13, atol=1e-10)
+    pos = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0]])
+    rad2, P2 = np.array([1.9, 1.6]), np.array([[0.6, -0.2, 0.0, 0.0]] * 2)
+    c2, pk2 = _all_pairs_list(2)
+    R1, R2, d = 3.3, 3.0, 0.1
+    s12 = 2 * math.pi * R1 * (R1 - d / 2 - (R1 ** 2 - R2 ** 2) / (2 * d))
+    assert s12 < 0
+    sasa_e, _ = O.lcpo_energies(pos, np.ones(2, np.int8), rad2, P2, np.full(2, 1e-21), c2, pk2)
+    _, sasa_f, _ = O.lcpo_forces(pos, np.ones(2, np.int8), rad2, P2, np.full(2, 1e-21), c2, pk2)
+    assert sasa_e[0] == pytest.approx(0.6 * 4 * math.pi * R1 ** 2, rel=1e-14)
+    assert sasa_f[0] == pytest.approx(0.6 * 4 * math.pi * R1 ** 2 - 0.2 * s12, rel=1e-14)
