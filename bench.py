@@ -246,6 +246,7 @@ def main():
     ap.add_argument("--waters", type=int, default=32768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gb", action="store_true", help="skip the GB/SA workloads")
+    ap.add_argument("--gb-only", action="store_true", help="only the GB/SA workloads (for profiling)")
     ap.add_argument("--gb-steps", type=int, default=200)
     ap.add_argument("--gb-systems", type=int, default=256)
     ap.add_argument("--gb-batch-steps", type=int, default=100)
@@ -270,6 +271,9 @@ def main():
     from pd_amd import units as U
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.gb_only:
+        print(json.dumps({"gbsa": gb_benchmarks(args, log)}), flush=True)
+        return
     t0 = time.perf_counter()
     w = S.water_box(args.waters, seed=2024)
     nb_kw = dict(Cutoff=10.0, InnerCutoff=8.0, VdwCutoff=10.0, VdwInnerCutoff=8.0, ForceSwitch=1)

@@ -338,6 +338,7 @@ int pdgpu_set_bonded(pdgpu_engine *h, int32_t nb, const int32_t *bi, const int32
       E.t_nn.upload(dvec(nn, 4 * nt, 1.0), E.stream);
       E.t_gamma.upload(dvec(gamma, 4 * nt, 1.0), E.stream);
     }
+    E.b_win.upload(pdgpu::bonded_windows(nb, bi, bj, na, ai, aa, aj, nt, ti, ta, tb, tj), E.stream);
     E.sync();
   });
 }

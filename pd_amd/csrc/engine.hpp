@@ -171,7 +171,14 @@ struct BondedArrays {
   const double *at0, *ak;
   const int *ti, *ta, *tb, *tj, *tterms;
   const double *tvn, *tnn, *tgamma;
+  const int2 *win;  // per block: first atom of its LDS force window and the window length (0: direct)
 };
+// Blocks of 256 terms never mix bonds, angles and torsions; a block whose terms
+// touch at most kBondedWindow consecutive atoms sums its forces in LDS first.
+constexpr int kBondedWindow = 1024;
+int bonded_blocks(int nb, int na, int nt);
+std::vector<int2> bonded_windows(int nb, const int *bi, const int *bj, int na, const int *ai, const int *aa,
+                                 const int *aj, int nt, const int *ti, const int *ta, const int *tb, const int *tj);
 void launch_bonded_kernel(hipStream_t s, const BondedArrays &B, const double *x, const double *y,
                           const double *z, long long *fx, long long *fy, long long *fz, double *partial,
                           double *eterm);
@@ -211,7 +218,8 @@ struct LCPOState {
   std::vector<int> h_atom_sys, h_first, h_count;
   DBuf<int8_t> use;
   DBuf<double> R, sig, nl_s, sasa, dvx, dvy, dvz, eatom;
-  DBuf<double4> P;
+  DBuf<double4> P, pr;  // P1..P4; packed position + radius (R < 0: not used)
+  double max_R = 0.0;
   DBuf<int> atom_sys, sys_first, sys_count, nl_cnt, nl_idx;
   void add_system(int first, int count, const int8_t *use, const double *radius, const double *P4,
                   const double *sigma_j);
@@ -301,6 +309,7 @@ struct Engine {
   // ---- bonded terms -------------------------------------------------------------
   int nbonds = 0, nangles = 0, ntors = 0;
   DBuf<int> b_i, b_j, a_i, a_a, a_j, t_i, t_a, t_b, t_j, t_terms;
+  DBuf<int2> b_win;
   DBuf<double> b_l, b_k, a_t0, a_k, t_vn, t_nn, t_gamma;
 
   explicit Engine(int natoms, int dev);
@@ -418,6 +427,16 @@ __device__ __forceinline__ void fix_add(long long *p, long long v) {
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
 }
 __device__ __forceinline__ float wave_sum_f(float v) {

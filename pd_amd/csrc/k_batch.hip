@@ -230,6 +230,7 @@ struct Batch {
   std::unique_ptr<LCPOState> lcpo;  // when every member has an LCPO component
   // bonded terms, global atom indices, numbered bonds | angles | torsions
   DBuf<int> bi, bj, ai, aa, aj, ti, ta, tb, tj, tterms;
+  DBuf<int2> bwin;
   DBuf<double> bl, bk, at0, ak, tvn, tnn, tgamma;
   int nb = 0, na = 0, nt = 0;
   DBuf<int> seg_beg[3], seg_end[3];  // per-system term ranges (bond, angle, torsion numbering)
@@ -264,7 +265,7 @@ struct Batch {
     if ((comps & PDGPU_GB) && have_gb) gb->run(stream, x.p, y.p, z.p, fx.p, fy.p, fz.p, e_sys.p, kStride, 0, nullptr);
     if ((comps & PDGPU_BONDED) && nb + na + nt > 0) {
       BondedArrays B{nb, na, nt, bi.p, bj.p, bl.p, bk.p, ai.p, aa.p, aj.p, at0.p, ak.p,
-                     ti.p, ta.p, tb.p, tj.p, tterms.p, tvn.p, tnn.p, tgamma.p};
+                     ti.p, ta.p, tb.p, tj.p, tterms.p, tvn.p, tnn.p, tgamma.p, bwin.p};
       launch_bonded_kernel(stream, B, x.p, y.p, z.p, fx.p, fy.p, fz.p, epart_dummy.p, eterm.p);
       for (int c = 0; c < 3; c++)
         hipLaunchKernelGGL(k_seg_sum, dim3(nsys), dim3(256), 0, stream, eterm.p, seg_beg[c].p, seg_end[c].p,
@@ -507,7 +508,10 @@ int pdgpu_batch_create(int32_t n_systems, pdgpu_engine *const *systems, pdgpu_ba
       b.seg_end[c].upload(se[c], st);
     }
     b.eterm.resize(std::max(1, b.nb + b.na + b.nt));
-    b.epart_dummy.resize(4 * (size_t)std::max(1, (b.nb + b.na + b.nt + 255) / 256));
+    b.epart_dummy.resize(4 * (size_t)pdgpu::bonded_blocks(b.nb, b.na, b.nt));
+    b.bwin.upload(pdgpu::bonded_windows(b.nb, Bi.data(), Bj.data(), b.na, Ai.data(), Aa.data(), Aj.data(), b.nt,
+                                        Ti.data(), Ta.data(), Tb.data(), Tj.data()),
+                  st);
     std::vector<int> ab(b.first), ae(n_systems);
     for (int s = 0; s < n_systems; s++) ae[s] = b.first[s] + b.count[s];
     b.atom_beg.upload(ab, st);
@@ -625,7 +629,8 @@ int pdgpu_batch_compute(pdgpu_batch *h, uint32_t components, double *f_accum, pd
     }
     if ((components & PDGPU_BONDED) && b.nb + b.na + b.nt > 0) {
       pdgpu::BondedArrays B{b.nb, b.na, b.nt, b.bi.p, b.bj.p, b.bl.p, b.bk.p, b.ai.p, b.aa.p, b.aj.p, b.at0.p,
-                            b.ak.p, b.ti.p, b.ta.p, b.tb.p, b.tj.p, b.tterms.p, b.tvn.p, b.tnn.p, b.tgamma.p};
+                            b.ak.p, b.ti.p, b.ta.p, b.tb.p, b.tj.p, b.tterms.p, b.tvn.p, b.tnn.p, b.tgamma.p,
+                            b.bwin.p};
       pdgpu::launch_bonded_kernel(b.stream, B, b.x.p, b.y.p, b.z.p, b.fx.p, b.fy.p, b.fz.p, b.epart_dummy.p,
                                   b.eterm.p);
       for (int c = 0; c < 3; c++)

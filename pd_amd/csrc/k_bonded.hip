@@ -3,19 +3,39 @@
 // One thread per term, PD's formulas (ffbonded.cpp:945-992 bonds, 1054-1177
 // angles, 1274-1512 dihedrals) in kcal/mol and Angstrom.  Forces go to the
 // original-order 64-bit fixed-point accumulators, energies to per-block FP64
-// partials reduced in a fixed order.
+// partials reduced in a fixed order.  The terms of a block of 256 (never
+// mixing kinds) usually touch a short run of consecutive atoms; the block then
+// sums its forces in an LDS window over that run and adds the window to global
+// memory once, with consecutive threads on consecutive atoms: global atomics
+// run at a bounded rate per lane operation on this GPU and scattered ones far
+// below it, so this replaces up to 12 scattered atomics per term by a few
+// coalesced ones per atom.
 #include "engine.hpp"
+
+#include <algorithm>
+#include <climits>
 
 namespace pdgpu {
 
 namespace {
 
-__device__ __forceinline__ void add3(long long *fx, long long *fy, long long *fz, int i, double gx, double gy,
-                                     double gz) {
-  fix_add(&fx[i], fix_from_double(gx));
-  fix_add(&fy[i], fix_from_double(gy));
-  fix_add(&fz[i], fix_from_double(gz));
-}
+struct ForceSink {
+  long long *fx, *fy, *fz;
+  long long *lds;  // 3 x kBondedWindow accumulators, or null
+  int lo;          // first atom of the window
+  __device__ __forceinline__ void add3(int i, double gx, double gy, double gz) const {
+    if (lds) {
+      unsigned long long *l = reinterpret_cast<unsigned long long *>(lds) + (i - lo);
+      atomicAdd(l, (unsigned long long)fix_from_double(gx));
+      atomicAdd(l + kBondedWindow, (unsigned long long)fix_from_double(gy));
+      atomicAdd(l + 2 * kBondedWindow, (unsigned long long)fix_from_double(gz));
+    } else {
+      fix_add(&fx[i], fix_from_double(gx));
+      fix_add(&fy[i], fix_from_double(gy));
+      fix_add(&fz[i], fix_from_double(gz));
+    }
+  }
+};
 
 __device__ void block_partial(double v0, double v1, double v2, double v3, double *out) {
   __shared__ double s[4][256];
@@ -42,8 +62,28 @@ __global__ __launch_bounds__(256) void k_bonded(
     const double *__restrict__ tnn, const double *__restrict__ tgamma, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, long long *__restrict__ fx,
     long long *__restrict__ fy, long long *__restrict__ fz, double *__restrict__ partial,
-    double *__restrict__ eterm) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double *__restrict__ eterm, const int2 *__restrict__ win) {
+  __shared__ long long acc[3 * kBondedWindow];
+  // block -> kind and term: bonds, then angles, then torsions, each padded to whole blocks
+  const int gb = (nb + 255) / 256, ga = (na + 255) / 256;
+  const int blk = blockIdx.x;
+  int t;
+  if (blk < gb) {
+    t = blk * 256 + (int)threadIdx.x;
+    if (t >= nb) t = nb + na + nt;  // idle thread
+  } else if (blk < gb + ga) {
+    t = (blk - gb) * 256 + (int)threadIdx.x;
+    t = t < na ? nb + t : nb + na + nt;
+  } else {
+    t = (blk - gb - ga) * 256 + (int)threadIdx.x;
+    t = t < nt ? nb + na + t : nb + na + nt;
+  }
+  const int2 w = win[blk];
+  const ForceSink F{fx, fy, fz, w.y > 0 ? acc : nullptr, w.x};
+  if (w.y > 0) {
+    for (int k = threadIdx.x; k < 3 * w.y; k += blockDim.x) acc[(k / w.y) * kBondedWindow + k % w.y] = 0;
+    __syncthreads();
+  }
   double eb = 0.0, ea = 0.0, et = 0.0, vir = 0.0;
   if (t < nb) {
     const int i = bi[t], j = bj[t];
@@ -57,8 +97,8 @@ __global__ __launch_bounds__(256) void k_bonded(
     vx *= s;
     vy *= s;
     vz *= s;
-    add3(fx, fy, fz, i, -vx, -vy, -vz);
-    add3(fx, fy, fz, j, vx, vy, vz);
+    F.add3(i, -vx, -vy, -vz);
+    F.add3(j, vx, vy, vz);
   } else if (t < nb + na) {
     const int q = t - nb;
     const double k = ak[q];
@@ -78,13 +118,13 @@ __global__ __launch_bounds__(256) void k_bonded(
       double bx = fm * inv_iv * (uix * cs - ujx) / sn;
       double by = fm * inv_iv * (uiy * cs - ujy) / sn;
       double bz = fm * inv_iv * (uiz * cs - ujz) / sn;
-      add3(fx, fy, fz, i, -bx, -by, -bz);
+      F.add3(i, -bx, -by, -bz);
       double ax = bx, ay = by, az = bz;
       bx = fm * inv_jv * (ujx * cs - uix) / sn;
       by = fm * inv_jv * (ujy * cs - uiy) / sn;
       bz = fm * inv_jv * (ujz * cs - uiz) / sn;
-      add3(fx, fy, fz, j, -bx, -by, -bz);
-      add3(fx, fy, fz, a, ax + bx, ay + by, az + bz);
+      F.add3(j, -bx, -by, -bz);
+      F.add3(a, ax + bx, ay + by, az + bz);
     }
   } else if (t < nb + na + nt) {
     const int q = t - nb - na;
@@ -173,34 +213,64 @@ __global__ __launch_bounds__(256) void k_bonded(
                           (2.0 * v2z * v1y - v1z * v2y) * c1y + c2y * v3x - c2x * v3y);
       }
     }
-    add3(fx, fy, fz, i0, fix, fiy, fiz);
-    add3(fx, fy, fz, i1, fabx - fix, faby - fiy, fabz - fiz);
-    add3(fx, fy, fz, i2, fjx - fabx, fjy - faby, fjz - fabz);
-    add3(fx, fy, fz, i3, -fjx, -fjy, -fjz);
+    F.add3(i0, fix, fiy, fiz);
+    F.add3(i1, fabx - fix, faby - fiy, fabz - fiz);
+    F.add3(i2, fjx - fabx, fjy - faby, fjz - fabz);
+    F.add3(i3, -fjx, -fjy, -fjz);
   }
   if (eterm && t < nb + na + nt) eterm[t] = eb + ea + et;
-  block_partial(eb, ea, et, vir, partial);
+  block_partial(eb, ea, et, vir, partial);  // also orders the LDS adds before the flush
+  if (w.y > 0) {
+    for (int k = threadIdx.x; k < 3 * w.y; k += blockDim.x) {
+      const int c = k / w.y, a = k % w.y;
+      const long long v = acc[c * kBondedWindow + a];
+      if (v != 0) fix_add(&(c == 0 ? fx : (c == 1 ? fy : fz))[w.x + a], v);
+    }
+  }
 }
 
 }  // namespace
 
+int bonded_blocks(int nb, int na, int nt) {
+  return std::max(1, (nb + 255) / 256 + (na + 255) / 256 + (nt + 255) / 256);
+}
+
+std::vector<int2> bonded_windows(int nb, const int *bi, const int *bj, int na, const int *ai, const int *aa,
+                                 const int *aj, int nt, const int *ti, const int *ta, const int *tb, const int *tj) {
+  std::vector<int2> w;
+  auto kind = [&](int cnt, std::initializer_list<const int *> idx) {
+    for (int b0 = 0; b0 < cnt; b0 += 256) {
+      int lo = INT32_MAX, hi = -1;
+      for (int t = b0; t < std::min(cnt, b0 + 256); t++)
+        for (const int *a : idx) {
+          lo = std::min(lo, a[t]);
+          hi = std::max(hi, a[t]);
+        }
+      w.push_back(hi - lo + 1 <= kBondedWindow ? make_int2(lo, hi - lo + 1) : make_int2(0, 0));
+    }
+  };
+  kind(nb, {bi, bj});
+  kind(na, {ai, aa, aj});
+  kind(nt, {ti, ta, tb, tj});
+  if (w.empty()) w.push_back(make_int2(0, 0));
+  return w;
+}
+
 void launch_bonded_kernel(hipStream_t s, const BondedArrays &B, const double *x, const double *y,
                           const double *z, long long *fx, long long *fy, long long *fz, double *partial,
                           double *eterm) {
-  const int total = B.nb + B.na + B.nt;
-  const int grid = std::max(1, (total + 255) / 256);
+  const int grid = bonded_blocks(B.nb, B.na, B.nt);
   hipLaunchKernelGGL(k_bonded, dim3(grid), dim3(256), 0, s, B.nb, B.bi, B.bj, B.bl, B.bk, B.na, B.ai, B.aa, B.aj,
                      B.at0, B.ak, B.nt, B.ti, B.ta, B.tb, B.tj, B.tterms, B.tvn, B.tnn, B.tgamma, x, y, z, fx, fy, fz,
-                     partial, eterm);
+                     partial, eterm, B.win);
 }
 
 void Engine::launch_bonded(double *energy_out) {
-  const int total = nbonds + nangles + ntors;
-  const int grid = std::max(1, (total + 255) / 256);
+  const int grid = bonded_blocks(nbonds, nangles, ntors);
   DBuf<double> &part = e_partial_bonded;
   part.resize((size_t)grid * 4);
   BondedArrays B{nbonds, nangles, ntors, b_i.p, b_j.p, b_l.p, b_k.p, a_i.p, a_a.p, a_j.p, a_t0.p, a_k.p,
-                 t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p, t_gamma.p};
+                 t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p, t_gamma.p, b_win.p};
   launch_bonded_kernel(stream, B, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, part.p, nullptr);
   PDGPU_HIP(hipGetLastError());
   reduce4(stream, part.p, grid, energy_out);

@@ -6,12 +6,12 @@
 // over the overlap neighbours j of i and the neighbours k of j that also
 // overlap i, with S_i = 4 pi R_i^2, R = LCPO radius + 1.4 A, and the energy
 // sum_i sigma_i SASA_i (lcpo.cpp:63).  Three kernels:
-//   k_lcpo_lists  one wavefront per atom: the overlap partners (d < R_i + R_j,
-//                 hydrogens excluded) in ascending index order, at most 80
-//                 (Nlistmax, lcpo.h:24) -- the order and truncation PD's fill
-//                 loop produces (lcpo.cpp:713-767), and the overlap
+//   k_lcpo_lists  one thread per atom over LDS tiles: the overlap partners
+//                 (d < R_i + R_j, hydrogens excluded) in ascending index order,
+//                 at most 80 (Nlistmax, lcpo.h:24) -- the order and truncation
+//                 PD's fill loop produces (lcpo.cpp:713-767), and the overlap
 //                 A_ij = 2 pi R_i (R_i - d/2 - (R_i^2 - R_j^2) / 2d);
-//   k_lcpo_sasa   one thread per atom: SASA_i and the derivative that PD
+//   k_lcpo_sasa   one wavefront per atom: SASA_i and the derivative that PD
 //                 collects on atom i (its own self terms plus the derivatives
 //                 of its neighbours' SASA with respect to x_i), clamped to zero
 //                 with the SASA when SASA_i < 0 (lcpo.cpp:940-944);
@@ -24,9 +24,12 @@
 // system, which is PD's neighbour list whenever that list holds every pair
 // within R_i + R_j (the all-pairs GB/SA setting, or any list of at least 7 A,
 // lcpo.cpp:37).  Everything is FP64 in PD's expression order, so the SASA and
-// forces reproduce the CPU arithmetic.
+// forces agree with the CPU arithmetic to FP64 rounding (the
+// sums are taken in a different order).
 #include "engine.hpp"
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 
 namespace pdgpu {
@@ -41,41 +44,71 @@ __device__ __forceinline__ bool lcpo_full(const int *step_ptr, int step_host, in
 
 // EONLY: the overlap lists of calcLCPOSasaEnergies (lcpo.cpp:469-520): the test
 // on the distance itself and sphereSurfaceOverlap clamped at zero.
+// One thread per atom i; the block stages tiles of 64 atoms in LDS and every
+// lane scans them in ascending index order, so each list comes out ascending
+// and the first kLcpoMax partners are kept, as PD's fill loop does.
 template <bool EONLY>
 __global__ __launch_bounds__(64) void k_lcpo_lists(int n, const int *__restrict__ atom_sys,
                                                    const int *__restrict__ sys_first,
-                                                   const int *__restrict__ sys_count, const double *__restrict__ x,
-                                                   const double *__restrict__ y, const double *__restrict__ z,
-                                                   const int8_t *__restrict__ use, const double *__restrict__ R,
+                                                   const int *__restrict__ sys_count, const double4 *__restrict__ pr,
                                                    const int *__restrict__ step_ptr, int step_host, int update,
-                                                   int *__restrict__ nl_cnt, int *__restrict__ nl_idx,
+                                                   double reach, int *__restrict__ nl_cnt, int *__restrict__ nl_idx,
                                                    double *__restrict__ nl_s) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x;
+  __shared__ double4 tp[64];
+  __shared__ int s_lo, s_hi;
+  if (!lcpo_full(step_ptr, step_host, update)) return;  // uniform: the step is the same for all
   const int lane = threadIdx.x;
-  if (i >= n || !lcpo_full(step_ptr, step_host, update)) return;
-  if (!use[i]) {
-    if (lane == 0) nl_cnt[i] = 0;
-    return;
+  const int i = blockIdx.x * 64 + lane;
+  const bool iv = i < n;
+  int first = 0, end = 0;
+  if (iv) {
+    const int s = atom_sys[i];
+    first = sys_first[s];
+    end = first + sys_count[s];
   }
-  const int s = atom_sys[i];
-  const int first = sys_first[s], count = sys_count[s];
-  const double xi = x[i], yi = y[i], zi = z[i], ri = R[i];
+  // the j range the block has to scan: the union of its atoms' systems
+  if (lane == 0) {
+    s_lo = INT32_MAX;
+    s_hi = 0;
+  }
+  __syncthreads();
+  if (iv) {
+    atomicMin(&s_lo, first);
+    atomicMax(&s_hi, end);
+  }
+  __syncthreads();
+  const int lo = s_lo, hi = s_hi;
+  const double4 pi = iv ? pr[i] : make_double4(0.0, 0.0, 0.0, -1.0);
+  const bool act = pi.w > 0.0;  // R < 0 marks atoms LCPO does not use
+  // bounding box of the block's active atoms, for skipping distant j tiles
+  const double bxl = wave_min(act ? pi.x : 1e300), bxh = wave_max(act ? pi.x : -1e300);
+  const double byl = wave_min(act ? pi.y : 1e300), byh = wave_max(act ? pi.y : -1e300);
+  const double bzl = wave_min(act ? pi.z : 1e300), bzh = wave_max(act ? pi.z : -1e300);
+  const double ri = pi.w;
   const double TwoPI = 2.0 * Consts::PI;
   int cnt = 0;
-  for (int base = 0; base < count && cnt < kLcpoMax; base += 64) {
-    const int j = first + base + lane;
-    bool ok = false;
-    double d2 = 0.0, rj = 0.0;
-    if (base + lane < count && j != i && use[j]) {
-      const double g = xi - x[j], h = yi - y[j], k = zi - z[j];
-      d2 = g * g + h * h + k * k;
-      rj = R[j];
-      ok = EONLY ? sqrt(d2) < (ri + rj) : d2 < (ri + rj) * (ri + rj);
-    }
-    const unsigned long long b = __ballot(ok);
-    const int pos = cnt + __popcll(b & ((1ull << lane) - 1ull));
-    if (ok && pos < kLcpoMax) {
+  for (int base = lo; base < hi; base += 64) {
+    const int jl = base + lane;
+    const double4 q = jl < hi ? pr[jl] : make_double4(1e300, 1e300, 1e300, -1.0);
+    // skip a tile whose active atoms all lie beyond reach (2 max R) of the block's box
+    const bool qa = q.w > 0.0;
+    const double gx = fmax(0.0, fmax(q.x - bxh, bxl - q.x)), gy = fmax(0.0, fmax(q.y - byh, byl - q.y)),
+                 gz = fmax(0.0, fmax(q.z - bzh, bzl - q.z));
+    if (!__ballot(qa && gx * gx + gy * gy + gz * gz <= reach * reach)) continue;
+    __syncthreads();
+    tp[lane] = q;
+    __syncthreads();
+    const int m = min(64, hi - base);
+    for (int t = 0; t < m; t++) {
+      const int j = base + t;
+      const double4 pj = tp[t];
+      if (!act || j < first || j >= end || j == i || !(pj.w > 0.0) || cnt >= kLcpoMax) continue;
+      const double g = pi.x - pj.x, h = pi.y - pj.y, k = pi.z - pj.z;
+      const double d2 = g * g + h * h + k * k;
+      const double rj = pj.w;
+      const bool ok = EONLY ? sqrt(d2) < (ri + rj) : d2 < (ri + rj) * (ri + rj);
+      if (!ok) continue;
       const double d = sqrt(d2);
       double sij;
       if (EONLY) {  // sphereSurfaceOverlap (maths.cpp:120-123), negative overlaps set to 0
@@ -83,127 +116,167 @@ __global__ __launch_bounds__(64) void k_lcpo_lists(int n, const int *__restrict_
         if (sij < 0) sij = 0;
       } else {
         const double inv = 1 / d;
-        const double t = 0.5 * (ri * ri - rj * rj) * inv;
-        sij = TwoPI * ri * (ri - d * 0.5 - t);
+        const double tt = 0.5 * (ri * ri - rj * rj) * inv;
+        sij = TwoPI * ri * (ri - d * 0.5 - tt);
       }
-      nl_idx[(size_t)i * kLcpoMax + pos] = j;
-      nl_s[(size_t)i * kLcpoMax + pos] = sij;
+      nl_idx[(size_t)i * kLcpoMax + cnt] = j;
+      nl_s[(size_t)i * kLcpoMax + cnt] = sij;
+      cnt++;
     }
-    cnt += __popcll(b);
   }
-  if (lane == 0) nl_cnt[i] = min(cnt, kLcpoMax);
+  if (iv) nl_cnt[i] = act ? cnt : 0;
 }
 
-// EONLY: calcLCPOSasaEnergies (lcpo.cpp:522-556): the SASA alone, k excluded
-// when (R_i + R_k) < d_ik.
+// Packs position and LCPO radius per atom (R < 0 for atoms LCPO does not use).
+__global__ void k_lcpo_pack(int n, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const int8_t *__restrict__ use,
+                            const double *__restrict__ R, const int *__restrict__ step_ptr, int step_host,
+                            int update, double4 *__restrict__ pr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !lcpo_full(step_ptr, step_host, update)) return;
+  pr[i] = make_double4(x[i], y[i], z[i], use[i] ? R[i] : -1.0);
+}
+
+// The SASA of atom i and the derivative PD collects on it (lcpo.cpp:786-948),
+// one wavefront per atom.  PD's loops over the overlap neighbours j of i and
+// the neighbours k of j are rearranged as one flat sum over the (j, k) pairs:
+// every k-term enters the derivative linearly, with weights that depend on j
+// alone (P3_j sigma_j, P4_j sigma_j, the overlap s_ji, the pair derivative of
+// s_ji), so the lanes take (j, k) pairs in turn after a first pass over j has
+// put those weights in LDS; the sums are reduced over the wave in a fixed
+// order.  EONLY: calcLCPOSasaEnergies (lcpo.cpp:522-556), the SASA alone, with
+// k excluded when (R_i + R_k) < d_ik.
 template <bool EONLY>
-__global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                  const double *__restrict__ z, const int8_t *__restrict__ use,
-                                                  const double *__restrict__ R, const double4 *__restrict__ P,
+__global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double4 *__restrict__ pr, const int8_t *__restrict__ use,
+                                                  const double4 *__restrict__ P,
                                                   const double *__restrict__ sig, const int *__restrict__ step_ptr,
                                                   int step_host, int update, const int *__restrict__ nl_cnt,
                                                   const int *__restrict__ nl_idx, const double *__restrict__ nl_s,
                                                   double *__restrict__ sasa, double *__restrict__ dvx,
                                                   double *__restrict__ dvy, double *__restrict__ dvz) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !lcpo_full(step_ptr, step_host, update)) return;
-  const double PI = Consts::PI, TwoPI = 2.0 * Consts::PI, FourPI = 4.0 * Consts::PI;
-  double der[3] = {0.0, 0.0, 0.0};
+  __shared__ int jidx[kLcpoMax], joff[kLcpoMax + 1];
+  __shared__ double jw[kLcpoMax][10];  // s_ji, P3 sigma, P4 sigma, Si2_t, d2(3), a2t(3)
+  const int lane = threadIdx.x;
+  if (!lcpo_full(step_ptr, step_host, update)) return;  // uniform: the step is the same for all
+  // a grid of a few thousand waves walks the atoms, so the launches on the
+  // steps that re-apply stored forces cost almost nothing
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+  __syncthreads();  // the LDS tables of the previous atom are no longer read
   if (!use[i]) {
-    sasa[i] = 0.0;
-    if (!EONLY) dvx[i] = dvy[i] = dvz[i] = 0.0;
-    return;
+    if (lane == 0) {
+      sasa[i] = 0.0;
+      if (!EONLY) dvx[i] = dvy[i] = dvz[i] = 0.0;
+    }
+    continue;
   }
-  const double pi[3] = {x[i], y[i], z[i]};
-  const double ri = R[i];
-  const double Si = FourPI * ri * ri;
-  double Si2 = 0, Si3 = 0, Si4 = 0;
-  double a2[3] = {0, 0, 0}, a4[3] = {0, 0, 0};
+  const double PI = Consts::PI, TwoPI = 2.0 * Consts::PI, FourPI = 4.0 * Consts::PI;
+  const double4 pi4 = pr[i];
+  const double pix = pi4.x, piy = pi4.y, piz = pi4.z;
+  const double ri = pi4.w;
   const int ni = nl_cnt[i];
-  for (int jn = 0; jn < ni; jn++) {
+  // pass over j: per-j weights and the k counts
+  double der[3] = {0, 0, 0}, a2[3] = {0, 0, 0}, a4[3] = {0, 0, 0};
+  double Si2 = 0, Si3 = 0, Si4 = 0;
+  for (int jn = lane; jn < ni; jn += 64) {
     const int ja = nl_idx[(size_t)i * kLcpoMax + jn];
-    const double pj[3] = {x[ja], y[ja], z[ja]};
     const double Si2_t = nl_s[(size_t)i * kLcpoMax + jn];
-    double Si3_t = 0, Si3ik_t = 0;
-    const double gx = pj[0] - pi[0], gy = pj[1] - pi[1], gz = pj[2] - pi[2];
-    const double dij = sqrt(gx * gx + gy * gy + gz * gz);
-    const double rj = R[ja];
-    const double inv = 1 / dij;
-    double t = 0.5 * (ri * ri - rj * rj) * inv;
-    const double sji = TwoPI * rj * (rj - dij * 0.5 + t);
-    t *= 2.0 * inv;
-    double d3[3] = {0, 0, 0}, d4[3], d3b[3] = {0, 0, 0}, d4b[3] = {0, 0, 0}, d2[3], a2t[3];
-    double dAdd = PI * rj * (-t - 1);
-    const double ddd[3] = {-(pi[0] - pj[0]) / dij, -(pi[1] - pj[1]) / dij, -(pi[2] - pj[2]) / dij};
-    for (int c = 0; c < 3; c++) d2[c] = dAdd * ddd[c];
-    dAdd = PI * ri * (t - 1);
-    for (int c = 0; c < 3; c++) a2t[c] = dAdd * ddd[c];
-    const int nj = nl_cnt[ja];
-    for (int kn = 0; kn < nj; kn++) {
-      const int ka = nl_idx[(size_t)ja * kLcpoMax + kn];
-      if (ka == i) continue;
-      const double pk[3] = {x[ka], y[ka], z[ka]};
-      const double rk = R[ka];
-      const double hx = pk[0] - pi[0], hy = pk[1] - pi[1], hz = pk[2] - pi[2];
-      double dik = hx * hx + hy * hy + hz * hz;
-      if (EONLY) {
-        if ((ri + rk) < sqrt(dik)) continue;
-        Si3_t += nl_s[(size_t)ja * kLcpoMax + kn];
-        continue;
-      }
-      if ((ri + rk) * (ri + rk) < dik) continue;
-      dik = sqrt(dik);
-      const double invik = 1 / dik;
-      const double u = (ri * ri - rk * rk) * (invik * invik);
-      double dA = PI * ri * (u - 1);
-      const double dk[3] = {-(pi[0] - pk[0]) * invik, -(pi[1] - pk[1]) * invik, -(pi[2] - pk[2]) * invik};
-      const double sjk = nl_s[(size_t)ja * kLcpoMax + kn];
-      for (int c = 0; c < 3; c++) d3[c] += dA * dk[c];
-      dA = PI * rk * (-u - 1);
-      Si3ik_t += PI * ri * (2 * ri - dik - dik * u);
-      for (int c = 0; c < 3; c++) d3b[c] += dA * dk[c];
-      for (int c = 0; c < 3; c++) d4b[c] += sjk * dA * dk[c];
-      Si3_t += sjk;
-    }
-    for (int c = 0; c < 3; c++) d4[c] = sji * d3[c] + d4b[c];
-    for (int c = 0; c < 3; c++) d3[c] += d3b[c];
-    for (int c = 0; c < 3; c++) d4[c] += d2[c] * Si3ik_t;
-    const double4 Pj = P[ja];
-    const double sj = sig[ja];
-    for (int c = 0; c < 3; c++) {
-      d2[c] *= Pj.y * sj;
-      d3[c] *= Pj.z * sj;
-      d4[c] *= Pj.w * sj;
-      der[c] += d2[c];
-      der[c] += d3[c];
-      der[c] += d4[c];
-    }
-    Si4 += (Si3_t * Si2_t);
-    Si3 += Si3_t;
+    jidx[jn] = ja;
+    joff[jn + 1] = nl_cnt[ja];
+    jw[jn][3] = Si2_t;
     Si2 += Si2_t;
-    for (int c = 0; c < 3; c++) {
-      a2[c] += a2t[c];
-      a4[c] += Si3_t * a2t[c];
+    if (!EONLY) {
+      const double4 pj4 = pr[ja];
+      const double pjx = pj4.x, pjy = pj4.y, pjz = pj4.z;
+      const double gx = pjx - pix, gy = pjy - piy, gz = pjz - piz;
+      const double dij = sqrt(gx * gx + gy * gy + gz * gz);
+      const double rj = pj4.w;
+      const double inv = 1 / dij;
+      double t = 0.5 * (ri * ri - rj * rj) * inv;
+      const double sji = TwoPI * rj * (rj - dij * 0.5 + t);
+      t *= 2.0 * inv;
+      const double ddd[3] = {-(pix - pjx) / dij, -(piy - pjy) / dij, -(piz - pjz) / dij};
+      const double dAj = PI * rj * (-t - 1), dAi = PI * ri * (t - 1);
+      const double4 Pj = P[ja];
+      const double sj = sig[ja];
+      jw[jn][0] = sji;
+      jw[jn][1] = Pj.z * sj;
+      jw[jn][2] = Pj.w * sj;
+      for (int c = 0; c < 3; c++) {
+        const double d2c = dAj * ddd[c];
+        jw[jn][4 + c] = d2c;
+        jw[jn][7 + c] = dAi * ddd[c];
+        der[c] += d2c * (Pj.y * sj);  // the P2 term of SASA_j
+        a2[c] += dAi * ddd[c];
+      }
     }
   }
-  const double4 Pi = P[i];
-  double SAi = Pi.x * Si + Pi.y * Si2 + Pi.z * Si3 + Pi.w * Si4;
-  for (int c = 0; c < 3; c++) {
-    a2[c] *= Pi.y * sig[i];
-    a4[c] *= Pi.w * sig[i];
-    der[c] += a2[c];
-    der[c] += a4[c];
+  __syncthreads();
+  if (lane == 0) {  // exclusive prefix of the k counts, in j order
+    joff[0] = 0;
+    for (int jn = 0; jn < ni; jn++) joff[jn + 1] += joff[jn];
   }
+  __syncthreads();
+  const int total = joff[ni];
+  int jn = 0;
+  for (int t = lane; t < total; t += 64) {
+    while (joff[jn + 1] <= t) jn++;  // t only grows, so j only moves forward
+    const int ja = jidx[jn];
+    const int kn = t - joff[jn];
+    const int ka = nl_idx[(size_t)ja * kLcpoMax + kn];
+    if (ka == i) continue;
+    const double4 pk4 = pr[ka];
+    const double pkx = pk4.x, pky = pk4.y, pkz = pk4.z;
+    const double rk = pk4.w;
+    const double hx = pkx - pix, hy = pky - piy, hz = pkz - piz;
+    double dik = hx * hx + hy * hy + hz * hz;
+    const double sjk = nl_s[(size_t)ja * kLcpoMax + kn];
+    if (EONLY) {
+      if ((ri + rk) < sqrt(dik)) continue;
+    } else {
+      if ((ri + rk) * (ri + rk) < dik) continue;
+    }
+    Si3 += sjk;
+    Si4 += jw[jn][3] * sjk;
+    if (EONLY) continue;
+    dik = sqrt(dik);
+    const double invik = 1 / dik;
+    const double u = (ri * ri - rk * rk) * (invik * invik);
+    const double dA1 = PI * ri * (u - 1);
+    const double dA2 = PI * rk * (-u - 1);
+    const double s3ik = PI * ri * (2 * ri - dik - dik * u);
+    const double dk[3] = {-(pix - pkx) * invik, -(piy - pky) * invik, -(piz - pkz) * invik};
+    const double c3 = jw[jn][1], c4 = jw[jn][2], sji = jw[jn][0];
+    for (int c = 0; c < 3; c++) {
+      der[c] += c3 * (dA1 * dk[c] + dA2 * dk[c]);                          // P3 term of SASA_j
+      der[c] += c4 * (sji * (dA1 * dk[c]) + sjk * dA2 * dk[c] + jw[jn][4 + c] * s3ik);  // P4 term of SASA_j
+      a4[c] += sjk * jw[jn][7 + c];
+    }
+  }
+  Si2 = wave_sum(Si2);
+  Si3 = wave_sum(Si3);
+  Si4 = wave_sum(Si4);
+  if (!EONLY)
+    for (int c = 0; c < 3; c++) {
+      der[c] = wave_sum(der[c]);
+      a2[c] = wave_sum(a2[c]);
+      a4[c] = wave_sum(a4[c]);
+    }
+  if (lane != 0) continue;
+  const double4 Pi = P[i];
+  const double Si = FourPI * ri * ri;
+  double SAi = Pi.x * Si + Pi.y * Si2 + Pi.z * Si3 + Pi.w * Si4;
+  for (int c = 0; c < 3; c++) der[c] += a2[c] * (Pi.y * sig[i]) + a4[c] * (Pi.w * sig[i]);
   if (SAi < 0) {  // LCPO can go negative for buried atoms (lcpo.cpp:940-944)
     SAi = 0;
     der[0] = der[1] = der[2] = 0.0;
   }
   sasa[i] = SAi;
-  if (EONLY) return;  // the stored derivatives of the last force evaluation stay
+  if (EONLY) continue;  // the stored derivatives of the last force evaluation stay
   dvx[i] = der[0];
   dvy[i] = der[1];
   dvz[i] = der[2];
+  }
 }
 
 // Adds the stored derivatives (J/A per molecule) as forces in fixed point
@@ -258,6 +331,9 @@ void LCPOState::append(const LCPOState &o, int first) {
 }
 
 void LCPOState::upload(hipStream_t s) {
+  max_R = 0.0;
+  for (int a = 0; a < n; a++)
+    if (h_use[a]) max_R = std::max(max_R, h_R[a]);
   use.upload(h_use, s);
   R.upload(h_R, s);
   std::vector<double4> p4(n);
@@ -267,6 +343,7 @@ void LCPOState::upload(hipStream_t s) {
   atom_sys.upload(h_atom_sys, s);
   sys_first.upload(h_first, s);
   sys_count.upload(h_count, s);
+  pr.resize(n);  // allocated here: run() may be recorded into a graph
   nl_cnt.resize(n);
   nl_idx.resize((size_t)n * kLcpoMax);
   nl_s.resize((size_t)n * kLcpoMax);
@@ -285,18 +362,21 @@ void LCPOState::upload(hipStream_t s) {
 void LCPOState::run(hipStream_t s, const double *x, const double *y, const double *z, long long *fx, long long *fy,
                     long long *fz, const int *step_ptr, int step_host, bool full, bool energy_only) {
   const int upd = (full || energy_only) ? 0 : update;
+  hipLaunchKernelGGL(k_lcpo_pack, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y, z, use.p, R.p, step_ptr,
+                     step_host, upd, pr.p);
+  const int lb = (n + 63) / 64;
   if (energy_only) {
-    hipLaunchKernelGGL(k_lcpo_lists<true>, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z,
-                       use.p, R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
-    hipLaunchKernelGGL(k_lcpo_sasa<true>, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p,
-                       step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
+    hipLaunchKernelGGL(k_lcpo_lists<true>, dim3(lb), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, pr.p,
+                       step_ptr, step_host, upd, 2.0 * max_R, nl_cnt.p, nl_idx.p, nl_s.p);
+    hipLaunchKernelGGL(k_lcpo_sasa<true>, dim3(std::min(n, 8192)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
+                       upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
     hipLaunchKernelGGL(k_lcpo_energy, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
                        sig.p, eatom.p);
   } else {
-    hipLaunchKernelGGL(k_lcpo_lists<false>, dim3(n), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, x, y, z,
-                       use.p, R.p, step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p);
-    hipLaunchKernelGGL(k_lcpo_sasa<false>, dim3((n + 63) / 64), dim3(64), 0, s, n, x, y, z, use.p, R.p, P.p, sig.p,
-                       step_ptr, step_host, upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
+    hipLaunchKernelGGL(k_lcpo_lists<false>, dim3(lb), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, pr.p,
+                       step_ptr, step_host, upd, 2.0 * max_R, nl_cnt.p, nl_idx.p, nl_s.p);
+    hipLaunchKernelGGL(k_lcpo_sasa<false>, dim3(std::min(n, 8192)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
+                       upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
     hipLaunchKernelGGL(k_lcpo_apply, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
                        sig.p, dvx.p, dvy.p, dvz.p, fx, fy, fz, eatom.p);
   }
