@@ -272,6 +272,8 @@ struct Engine {
   double grid_lo[3] = {0, 0, 0}, grid_w[2] = {0, 0};
   int max_clusters = 0;                    // allocation bound
   DBuf<double> wx, wy, wz;                 // wrapped coordinates used for binning
+  DBuf<int> atom_wrap;                     // binning wrap counts, one byte per axis
+  DBuf<unsigned char> slot_list;           // per-slot pair-search records (k_list.hip)
   DBuf<int> atom_col, col_count, col_fill, col_atom_start, col_cl_start, col_ncl;
   DBuf<int> col_atoms;                     // atoms grouped by column (unsorted)
   DBuf<int> slot_orig, orig_slot;

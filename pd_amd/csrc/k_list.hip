@@ -23,7 +23,8 @@ namespace pdgpu {
 
 namespace {
 
-constexpr int kMaxCand = 1536;  // candidate j-clusters per i-cluster (LDS)
+constexpr int kMaxCand = 1536;  // candidate j-clusters per work group (LDS)
+constexpr int kCandPart = 512;  // candidates found by one wave of the group
 
 __global__ void k_clean_space(int n_mol, const int *__restrict__ first, const int *__restrict__ last,
                               double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
@@ -97,27 +98,58 @@ __device__ __forceinline__ double wrap_coord(double v, double L) {
   return v - L * floor(v / L + 0.5);
 }
 
+// Adds 1 for every lane to counter[key] with one atomic per distinct key in the
+// wave (atoms of a wave are usually neighbours, so their keys mostly agree);
+// returns each lane's rank among the lanes that share its key plus the base
+// the atomic returned, i.e. a slot unique to the lane.
+__device__ __forceinline__ int wave_aggregated_add(int *counter, int key, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  int slot = 0;
+  while (pending) {  // each round retires at least the leader's key
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lk = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(active && key == lk) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counter[lk], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return slot;
+}
+
 __global__ void k_bin(int n, GridArgs g, const double *__restrict__ x, const double *__restrict__ y,
                       const double *__restrict__ z, double *__restrict__ wx, double *__restrict__ wy,
-                      double *__restrict__ wz, int *__restrict__ atom_col, int *__restrict__ col_count) {
+                      double *__restrict__ wz, int *__restrict__ wrap, int *__restrict__ atom_col,
+                      int *__restrict__ col_count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a = x[i], b = y[i], c = z[i];
-  if (g.periodic) {
-    a = wrap_coord(a, g.L[0]);
-    b = wrap_coord(b, g.L[1]);
-    c = wrap_coord(c, g.L[2]);
+  const bool v = i < n;
+  int col = 0;
+  if (v) {
+    double a = x[i], b = y[i], c = z[i];
+    int k[3] = {0, 0, 0};
+    if (g.periodic) {
+      // wrapped = v + k L with k = -floor(v / L + 1/2): the representative in [-L/2, L/2)
+      k[0] = -(int)floor(a / g.L[0] + 0.5);
+      k[1] = -(int)floor(b / g.L[1] + 0.5);
+      k[2] = -(int)floor(c / g.L[2] + 0.5);
+      a = wrap_coord(a, g.L[0]);
+      b = wrap_coord(b, g.L[1]);
+      c = wrap_coord(c, g.L[2]);
+    }
+    wx[i] = a;
+    wy[i] = b;
+    wz[i] = c;
+    wrap[i] = (k[0] & 0xFF) | ((k[1] & 0xFF) << 8) | ((k[2] & 0xFF) << 16);
+    int cx = (int)floor((a - g.lo_x) / g.w_x);
+    int cy = (int)floor((b - g.lo_y) / g.w_y);
+    cx = min(max(cx, 0), g.ncx - 1);
+    cy = min(max(cy, 0), g.ncy - 1);
+    col = cy * g.ncx + cx;
+    atom_col[i] = col;
   }
-  wx[i] = a;
-  wy[i] = b;
-  wz[i] = c;
-  int cx = (int)floor((a - g.lo_x) / g.w_x);
-  int cy = (int)floor((b - g.lo_y) / g.w_y);
-  cx = min(max(cx, 0), g.ncx - 1);
-  cy = min(max(cy, 0), g.ncy - 1);
-  int col = cy * g.ncx + cx;
-  atom_col[i] = col;
-  atomicAdd(&col_count[col], 1);
+  (void)wave_aggregated_add(col_count, col, v);
 }
 
 // Single-workgroup exclusive scans over the columns.
@@ -189,34 +221,49 @@ __global__ void k_fill_groups(int ncol, const int *__restrict__ col_cl_start, co
 __global__ void k_fill_columns(int n, const int *__restrict__ atom_col, const int *__restrict__ col_atom_start,
                                int *__restrict__ col_fill, int *__restrict__ col_atoms) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int c = atom_col[i];
-  int k = atomicAdd(&col_fill[c], 1);
-  col_atoms[col_atom_start[c] + k] = i;
+  const bool v = i < n;
+  const int c = v ? atom_col[i] : 0;
+  const int k = wave_aggregated_add(col_fill, c, v);  // order inside a column is fixed by the sort below
+  if (v) col_atoms[col_atom_start[c] + k] = i;
 }
 
 // Rank of each atom inside its column by (wrapped z, atom index): a total order,
-// so the slot assignment does not depend on the atomic fill order.
-__global__ void k_sort_columns(int n, const int *__restrict__ atom_col, const int *__restrict__ col_count,
-                               const int *__restrict__ col_atom_start, const int *__restrict__ col_cl_start,
-                               const int *__restrict__ col_atoms, const double *__restrict__ wz,
-                               int *__restrict__ slot_orig, int *__restrict__ orig_slot,
-                               int *__restrict__ cl_col) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  const int c = atom_col[a];
-  const int s = col_atom_start[c], cnt = col_count[c];
-  const double za = wz[a];
-  int rank = 0;
-  for (int k = 0; k < cnt; k++) {
-    int b = col_atoms[s + k];
-    double zb = wz[b];
-    rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
+// so the slot assignment does not depend on the atomic fill order.  One
+// workgroup per column; columns of up to kSortLds atoms are ranked from LDS.
+constexpr int kSortLds = 2048;
+__global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__restrict__ col_count,
+                                                      const int *__restrict__ col_atom_start,
+                                                      const int *__restrict__ col_cl_start,
+                                                      const int *__restrict__ col_atoms, const double *__restrict__ wz,
+                                                      int *__restrict__ slot_orig, int *__restrict__ orig_slot,
+                                                      int *__restrict__ cl_col) {
+  __shared__ double sz[kSortLds];
+  __shared__ int sa[kSortLds];
+  const int c = blockIdx.x;
+  if (c >= ncol) return;
+  const int s0 = col_atom_start[c], cnt = col_count[c];
+  const bool in_lds = cnt <= kSortLds;
+  if (in_lds)
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+      const int a = col_atoms[s0 + k];
+      sa[k] = a;
+      sz[k] = wz[a];
+    }
+  __syncthreads();
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const int a = in_lds ? sa[k] : col_atoms[s0 + k];
+    const double za = in_lds ? sz[k] : wz[a];
+    int rank = 0;
+    for (int m = 0; m < cnt; m++) {
+      const int b = in_lds ? sa[m] : col_atoms[s0 + m];
+      const double zb = in_lds ? sz[m] : wz[b];
+      rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
+    }
+    const int slot = col_cl_start[c] * kClusterSize + rank;
+    slot_orig[slot] = a;
+    orig_slot[a] = slot;
+    if ((rank & (kClusterSize - 1)) == 0) cl_col[slot / kClusterSize] = c;
   }
-  int slot = col_cl_start[c] * kClusterSize + rank;
-  slot_orig[slot] = a;
-  orig_slot[a] = slot;
-  cl_col[slot / kClusterSize] = c;
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int v, int count) {
@@ -244,6 +291,42 @@ __global__ void k_cluster_bbox(const int *__restrict__ n_clusters, int max_clust
   }
 }
 
+// Per-slot data for the pair search: the binning's wrapped position, the atom
+// index, the wrap counts (one signed byte per axis: wrapped = PD position +
+// count * box) and whether the atom has sparse bond-order partners (bit 24).
+struct SlotList {
+  double wx, wy, wz;
+  int orig, aux;
+};
+
+__global__ void k_slot_pack(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
+                            const double *__restrict__ wx, const double *__restrict__ wy,
+                            const double *__restrict__ wz, const int *__restrict__ wrap,
+                            const int *__restrict__ sp_off, SlotList *__restrict__ sl) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots || s >= *n_clusters * kClusterSize) return;
+  const int a = slot_orig[s];
+  SlotList v;
+  if (a >= 0) {
+    v.wx = wx[a];
+    v.wy = wy[a];
+    v.wz = wz[a];
+    v.orig = a;
+    v.aux = wrap[a] | ((sp_off[a + 1] > sp_off[a]) ? (1 << 24) : 0);
+  } else {
+    v.wx = v.wy = v.wz = 0.0;
+    v.orig = -1;
+    v.aux = 0;
+  }
+  sl[s] = v;
+}
+
+__device__ __forceinline__ int wrap_digit(int aux, int axis) { return (int)(int8_t)((aux >> (8 * axis)) & 0xFF); }
+// image index contribution of the wrap counts: 9 kx + 3 ky + kz
+__device__ __forceinline__ int wrap_image(int aux) {
+  return 9 * wrap_digit(aux, 0) + 3 * wrap_digit(aux, 1) + wrap_digit(aux, 2);
+}
+
 struct SearchArgs {
   int periodic;
   int ncx, ncy;
@@ -251,20 +334,23 @@ struct SearchArgs {
   double L[3];
   double half[3];
   double rlist2;                // (Cutoff+Padding)^2, PD's sqrIncludeLimit
+  double rlist2_lo, rlist2_hi;  // band around rlist2 resolved with PD's own arithmetic
   double Rbox2;                 // squared bbox pruning radius (rlist plus a margin)
+  int fast;                     // every pair inside rlist is its own minimum image
   int mid;                      // BondOrder::m_MaxIndexDelta
   int stride;                   // entries reserved per i-cluster
 };
 
-__device__ __forceinline__ int bond_code(int oa, int ob, int mid, const int8_t *__restrict__ band,
+__device__ __forceinline__ int bond_code(int oa, int ob, bool lo_sparse, int mid, const int8_t *__restrict__ band,
                                          const int *__restrict__ sp_off, const int *__restrict__ sp_part,
                                          const int *__restrict__ sp_ord) {
   int hi = max(oa, ob), lo = min(oa, ob);
   int bo = 7;
   if (hi - lo < mid) bo = band[(long)lo * mid + (hi - lo - 1)];
   // off-diagonal corrections (neighbourlist.cpp:179-220): the last partner entry wins
-  for (int k = sp_off[lo]; k < sp_off[lo + 1]; k++)
-    if (sp_part[k] == hi) bo = sp_ord[k];
+  if (lo_sparse)
+    for (int k = sp_off[lo]; k < sp_off[lo + 1]; k++)
+      if (sp_part[k] == hi) bo = sp_ord[k];
   bo &= 7;
   return bo == 1 ? 1 : bo == 2 ? 2 : bo == 3 ? 3 : 0;
 }
@@ -280,19 +366,24 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
     SearchArgs A, const int *__restrict__ n_groups, const int *__restrict__ grp_first,
     const int *__restrict__ grp_count, const int *__restrict__ cl_col, const int *__restrict__ col_cl_start,
     const int *__restrict__ col_ncl, const double *__restrict__ cl_lo, const double *__restrict__ cl_hi,
-    const int *__restrict__ slot_orig, const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, const double *__restrict__ wx, const double *__restrict__ wy,
-    const double *__restrict__ wz, const int8_t *__restrict__ band, const int *__restrict__ sp_off,
+    const SlotList *__restrict__ sl, const double *__restrict__ px, const double *__restrict__ py,
+    const double *__restrict__ pz, const int8_t *__restrict__ band, const int *__restrict__ sp_off,
     const int *__restrict__ sp_part, const int *__restrict__ sp_ord, uint2 *__restrict__ entries,
     int *__restrict__ cl_entry_count, int *__restrict__ wg_ncand, int *__restrict__ wg_cand,
     unsigned long long *__restrict__ counter) {
 #pragma clang fp contract(off)
-  __shared__ int cand_cl[kMaxCand];
-  __shared__ int cand_sh[kMaxCand];
-  __shared__ int s_ncand;
+  // phase 1 fills the two halves with the waves' candidate parts; phase 2
+  // reuses the space for each wave's list of candidates near its own cluster
+  __shared__ int cand[2 * kGroup * kCandPart];
+  __shared__ int join_cl[kMaxCand];
+  __shared__ unsigned char join_sh[kMaxCand];
+  __shared__ SlotList s_i[kGroup][kClusterSize];
+  __shared__ int part_n[kGroup];
+  __shared__ int s_ncand, s_over;
+  int *const cand_cl = cand, *const cand_sh = cand + kGroup * kCandPart;
   const int g = blockIdx.x;
   if (g >= *n_groups) return;  // the whole workgroup leaves together
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int c0 = grp_first[g], ng = grp_count[g];
   double glo[3], ghi[3];  // bounding box of the group's clusters
   for (int d = 0; d < 3; d++) {
@@ -307,19 +398,18 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   const int col = cl_col[c0];
   const int cx = col % A.ncx, cy = col / A.ncx;
 
-  // ---- phase 1 (first wave): candidate j-clusters in a fixed order -----------
+  // ---- phase 1: candidate j-clusters in a fixed order ------------------------
   // The group's own clusters come first (ranks 0..ng-1), then every cluster of
   // the column stencil, image by image, whose box comes within reach of the
   // group's box.  Only j-clusters >= the group's first cluster are candidates
-  // (each unordered cluster pair is searched from its lower cluster).
-  if (wave == 0) {
-    if (lane < ng) {
-      cand_cl[lane] = c0 + lane;
-      cand_sh[lane] = 13;
-    }
-    int ncand = ng;
+  // (each unordered cluster pair is searched from its lower cluster).  The
+  // stencil rows are dealt to the waves round-robin; each wave fills its own
+  // part of the LDS table and the parts are joined in wave order.
+  {
     const int kz_lo = A.periodic ? -1 : 0, kz_hi = A.periodic ? 1 : 0;
-    for (int dy = -A.Dy; dy <= A.Dy; dy++) {
+    int nw = 0;  // this wave's candidates
+    int *my_cl = cand_cl + wave * kCandPart, *my_sh = cand_sh + wave * kCandPart;
+    for (int dy = -A.Dy + wave; dy <= A.Dy; dy += kGroup) {
       int uy = cy + dy, ky = 0;
       if (A.periodic) {
         for (int it = 0; it < 4 && uy < 0; it++) { uy += A.ncy; ky -= 1; }
@@ -355,113 +445,180 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
             }
             const unsigned long long b = __ballot(ok);
             const int pre = __popcll(b & ((1ull << lane) - 1ull));
-            if (ok) {
-              const int pos = ncand + pre;
-              if (pos < kMaxCand) {
-                cand_cl[pos] = cj;
-                cand_sh[pos] = sh;
-              }
+            if (ok && nw + pre < kCandPart) {
+              my_cl[nw + pre] = cj;
+              my_sh[nw + pre] = sh;
             }
-            ncand += __popcll(b);
+            nw += __popcll(b);
           }
         }
       }
     }
-    if (lane == 0) s_ncand = ncand;
+    if (lane == 0) part_n[wave] = nw;
   }
   __syncthreads();
-  const bool cand_overflow = s_ncand > kMaxCand;
+  if (threadIdx.x == 0) {  // join: own clusters, then the waves' parts in order
+    int tot = ng, over = 0;
+    for (int w = 0; w < kGroup; w++) {
+      over |= part_n[w] > kCandPart;
+      tot += min(part_n[w], kCandPart);
+    }
+    s_ncand = tot;
+    s_over = over || tot > kMaxCand;
+  }
+  __syncthreads();
+  {
+    int off = ng;
+    for (int w = 0; w < wave; w++) off += min(part_n[w], kCandPart);
+    const int m = min(part_n[wave], kCandPart);
+    // move this wave's part after the preceding ones (a separate join buffer keeps the reads and writes apart)
+    for (int k = lane; k < m; k += 64) {
+      if (off + k < kMaxCand) {
+        join_cl[off + k] = cand_cl[wave * kCandPart + k];
+        join_sh[off + k] = (unsigned char)cand_sh[wave * kCandPart + k];
+      }
+    }
+    if (wave == 0 && lane < ng) {
+      join_cl[lane] = c0 + lane;
+      join_sh[lane] = 13;
+    }
+    if (wave < ng && lane < kClusterSize) s_i[wave][lane] = sl[(c0 + wave) * kClusterSize + lane];
+  }
+  __syncthreads();
+  const bool cand_overflow = s_over != 0;
   const int ncand = min(s_ncand, kMaxCand);
   // the candidate table the nonbonded kernel mirrors in LDS (ranks < kGroupRanks)
   const int nr = min(ncand, kGroupRanks);
-  for (int t = threadIdx.x; t < nr; t += blockDim.x) wg_cand[(size_t)g * kGroupStride + t] = cand_cl[t];
+  for (int t = threadIdx.x; t < nr; t += blockDim.x) wg_cand[(size_t)g * kGroupStride + t] = join_cl[t];
   if (threadIdx.x == 0) wg_ncand[g] = nr;
   if (wave >= ng) return;
 
-  // ---- phase 2 (one wave per i-cluster): exact PD pair tests, 8 x 8 per candidate
+  // ---- phase 2a: this wave's cluster against the candidates' boxes -----------
   const int ci = c0 + wave;
-  const double ilo0 = cl_lo[3 * ci], ilo1 = cl_lo[3 * ci + 1], ilo2 = cl_lo[3 * ci + 2];
-  const double ihi0 = cl_hi[3 * ci], ihi1 = cl_hi[3 * ci + 1], ihi2 = cl_hi[3 * ci + 2];
-  const int ii = lane & 7, jj = lane >> 3;
-  const int sa = ci * kClusterSize + ii;
-  const int oa = slot_orig[sa];
-  double pax = 0, pay = 0, paz = 0, wax = 0, way = 0, waz = 0;
-  if (oa >= 0) {
-    pax = x[oa]; pay = y[oa]; paz = z[oa];
-    wax = wx[oa]; way = wy[oa]; waz = wz[oa];
+  constexpr int kNearPerWave = 2 * kCandPart * 2;  // 16-bit slots of the phase-1 space per wave
+  static_assert(kNearPerWave >= kMaxCand, "a wave's near list must hold every candidate");
+  unsigned short *near = reinterpret_cast<unsigned short *>(cand) + wave * kNearPerWave;
+  int nnear = 0;
+  {
+    const double ilo0 = cl_lo[3 * ci], ilo1 = cl_lo[3 * ci + 1], ilo2 = cl_lo[3 * ci + 2];
+    const double ihi0 = cl_hi[3 * ci], ihi1 = cl_hi[3 * ci + 1], ihi2 = cl_hi[3 * ci + 2];
+    for (int base = 0; base < ncand; base += 64) {
+      const int k = base + lane;
+      bool ok = false;
+      if (k < ncand) {
+        const int cj = join_cl[k];
+        if (cj >= ci) {  // searched from the lower cluster
+          const int csh = join_sh[k];
+          const double shx = (csh / 9 - 1) * A.L[0], shy = ((csh / 3) % 3 - 1) * A.L[1],
+                       shz = (csh % 3 - 1) * A.L[2];
+          double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ihi0, ilo0 - (cl_hi[3 * cj] + shx)));
+          double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ihi1, ilo1 - (cl_hi[3 * cj + 1] + shy)));
+          double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ihi2, ilo2 - (cl_hi[3 * cj + 2] + shz)));
+          ok = (g0 * g0 + g1 * g1 + g2 * g2) <= A.Rbox2;
+        }
+      }
+      const unsigned long long b = __ballot(ok);
+      if (ok) near[nnear + __popcll(b & ((1ull << lane) - 1ull))] = (unsigned short)k;
+      nnear += __popcll(b);
+    }
   }
+  // the list is read back by other lanes of the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // ---- phase 2b: exact PD pair tests, lane = (candidate, j atom), loop over i atoms
+  // A pair is found through exactly one (cluster, shift) candidate: the one
+  // whose shift is PD's minimum image less the difference of the binning's
+  // wrap counts.  Pairs clearly inside or outside rlist are decided from the
+  // wrapped positions (their rounding error is orders of magnitude below the
+  // band); pairs in the band, and every pair of a box too small for the minimum
+  // image to be implied, take PD's own arithmetic on its positions.
   uint2 *E = entries + (size_t)ci * A.stride;
+  const int jj = lane & 7;
   int cnt = 0;
-  for (int k = 0; k < ncand; k++) {
-    const int cj = cand_cl[k];
-    const int csh = cand_sh[k];
-    if (cj < ci) continue;  // searched from the lower cluster
-    {  // this cluster's own box against the candidate (uniform over the wave)
-      const double shx = (csh / 9 - 1) * A.L[0], shy = ((csh / 3) % 3 - 1) * A.L[1], shz = (csh % 3 - 1) * A.L[2];
-      double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ihi0, ilo0 - (cl_hi[3 * cj] + shx)));
-      double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ihi1, ilo1 - (cl_hi[3 * cj + 1] + shy)));
-      double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ihi2, ilo2 - (cl_hi[3 * cj + 2] + shz)));
-      if ((g0 * g0 + g1 * g1 + g2 * g2) > A.Rbox2) continue;
-    }
-    const unsigned rank = (unsigned)min(k, kGroupRanks);
+  for (int b0 = 0; b0 < nnear; b0 += 8) {
+    const int t = b0 + (lane >> 3);
+    const bool have = t < nnear;
+    const int k = have ? (int)near[t] : 0;
+    const int cj = have ? join_cl[k] : ci;
+    const int csh = have ? (int)join_sh[k] : 13;
+    const int sx = csh / 9 - 1, sy = (csh / 3) % 3 - 1, sz = csh % 3 - 1;
+    const double shx = sx * A.L[0], shy = sy * A.L[1], shz = sz * A.L[2];
     const int sb = cj * kClusterSize + jj;
-    const int ob = slot_orig[sb];
-    bool in = (oa >= 0) && (ob >= 0) && (cj != ci || sa < sb);
-    int img = 13;
-    int code = 0;
-    if (in) {
-      // dedup across candidate shifts: the wrapped-frame minimum image of this
-      // pair must be the candidate's shift
-      if (A.periodic) {
-        double ex = wx[ob] - wax, ey = wy[ob] - way, ez = wz[ob] - waz;
-        int kxw = 0, kyw = 0, kzw = 0;
-        for (int it = 0; it < 8 && ex > A.half[0]; it++) { ex -= A.L[0]; kxw -= 1; }
-        for (int it = 0; it < 8 && ex < -A.half[0]; it++) { ex += A.L[0]; kxw += 1; }
-        for (int it = 0; it < 8 && ey > A.half[1]; it++) { ey -= A.L[1]; kyw -= 1; }
-        for (int it = 0; it < 8 && ey < -A.half[1]; it++) { ey += A.L[1]; kyw += 1; }
-        for (int it = 0; it < 8 && ez > A.half[2]; it++) { ez -= A.L[2]; kzw -= 1; }
-        for (int it = 0; it < 8 && ez < -A.half[2]; it++) { ez += A.L[2]; kzw += 1; }
-        in = ((kxw + 1) * 9 + (kyw + 1) * 3 + (kzw + 1)) == csh;
+    const SlotList lj = sl[sb];
+    const int ob = lj.orig;
+    const double jx = lj.wx + shx, jy = lj.wy + shy, jz = lj.wz + shz;
+    const int jimg = 13 + 9 * sx + 3 * sy + sz + wrap_image(lj.aux);
+    unsigned lm = 0, codes = 0;
+    unsigned long long imgs = 0;  // 5 bits per i atom
+    const bool jok = have && ob >= 0;
+#pragma unroll 1
+    for (int ii = 0; ii < kClusterSize; ii++) {
+      const SlotList li = s_i[wave][ii];
+      const int oa = li.orig;
+      bool in = jok && oa >= 0 && (cj != ci || ii < jj);
+      int img = jimg - wrap_image(li.aux);
+      if (in) {
+        const double dx = jx - li.wx, dy = jy - li.wy, dz = jz - li.wz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (A.fast && r2 < A.rlist2_lo) {
+          in = true;
+        } else if (A.fast && r2 > A.rlist2_hi) {
+          in = false;
+        } else {
+          // PD's arithmetic: dc = p_j - p_i, while-loop wrapping (neighbourlist.cpp:383-392)
+          double ex = px[ob] - px[oa], ey = py[ob] - py[oa], ez = pz[ob] - pz[oa];
+          int pimg = 13;
+          if (A.periodic) {
+            for (int it = 0; it < 8 && ex > A.half[0]; it++) { ex -= A.L[0]; pimg -= 9; }
+            for (int it = 0; it < 8 && ex < -A.half[0]; it++) { ex += A.L[0]; pimg += 9; }
+            for (int it = 0; it < 8 && ey > A.half[1]; it++) { ey -= A.L[1]; pimg -= 3; }
+            for (int it = 0; it < 8 && ey < -A.half[1]; it++) { ey += A.L[1]; pimg += 3; }
+            for (int it = 0; it < 8 && ez > A.half[2]; it++) { ez -= A.L[2]; pimg -= 1; }
+            for (int it = 0; it < 8 && ez < -A.half[2]; it++) { ez += A.L[2]; pimg += 1; }
+            const int mx = pimg / 9 - 1, my = (pimg / 3) % 3 - 1, mz = pimg % 3 - 1;
+            in = (mx - (wrap_digit(lj.aux, 0) - wrap_digit(li.aux, 0))) == sx &&
+                 (my - (wrap_digit(lj.aux, 1) - wrap_digit(li.aux, 1))) == sy &&
+                 (mz - (wrap_digit(lj.aux, 2) - wrap_digit(li.aux, 2))) == sz;
+          }
+          const double er2 = ex * ex + ey * ey + ez * ez;
+          in = in && er2 <= A.rlist2;
+          img = pimg;
+        }
+      }
+      if (in) {
+        lm |= 1u << ii;
+        imgs |= (unsigned long long)(img & 31) << (5 * ii);
+        const int lo_aux = oa < ob ? li.aux : lj.aux;
+        if (abs(oa - ob) < A.mid || ((lo_aux >> 24) & 1)) {
+          const int code = bond_code(oa, ob, (lo_aux >> 24) & 1, A.mid, band, sp_off, sp_part, sp_ord);
+          codes |= (unsigned)code << (2 * ii);
+        }
       }
     }
-    if (in) {
-      // PD's arithmetic: dc = p_j - p_i, while-loop wrapping (neighbourlist.cpp:383-392)
-      double dx = x[ob] - pax, dy = y[ob] - pay, dz = z[ob] - paz;
-      if (A.periodic) {
-        for (int it = 0; it < 8 && dx > A.half[0]; it++) { dx -= A.L[0]; img -= 9; }
-        for (int it = 0; it < 8 && dx < -A.half[0]; it++) { dx += A.L[0]; img += 9; }
-        for (int it = 0; it < 8 && dy > A.half[1]; it++) { dy -= A.L[1]; img -= 3; }
-        for (int it = 0; it < 8 && dy < -A.half[1]; it++) { dy += A.L[1]; img += 3; }
-        for (int it = 0; it < 8 && dz > A.half[2]; it++) { dz -= A.L[2]; img -= 1; }
-        for (int it = 0; it < 8 && dz < -A.half[2]; it++) { dz += A.L[2]; img += 1; }
-      }
-      double r2 = dx * dx + dy * dy + dz * dz;
-      in = r2 <= A.rlist2;
-      if (in) code = bond_code(oa, ob, A.mid, band, sp_off, sp_part, sp_ord);
-    }
-    // emit one entry per (j atom, image); usually one round
-    unsigned long long remaining = __ballot(in);
-    while (remaining) {  // each round clears at least one bit
-      unsigned grp = (unsigned)((remaining >> (jj * 8)) & 0xFFull);
-      int leader = grp ? (jj * 8 + __ffs(grp) - 1) : lane;
-      int limg = __shfl(img, leader, 64);
-      bool match = in && ((remaining >> lane) & 1ull) && img == limg;
-      unsigned long long mb = __ballot(match);
-      unsigned long long lob = __ballot(match && (code & 1));
-      unsigned long long hib = __ballot(match && (code & 2));
-      unsigned gmask = 0;
+    // one entry per (j atom, image); the i atoms of a lane usually share one image
+    const unsigned rank = (unsigned)min(k, kGroupRanks);
+    unsigned rem = lm;
+    while (__ballot(rem != 0)) {
+      const bool has = rem != 0;
+      unsigned m = 0;
+      int limg = 0;
+      if (has) {
+        const int l = __ffs(rem) - 1;
+        limg = (int)((imgs >> (5 * l)) & 31ull);
 #pragma unroll
-      for (int q = 0; q < 8; q++) gmask |= (((mb >> (q * 8)) & 0xFFull) ? 1u : 0u) << q;
-      if (ii == 0 && ((gmask >> jj) & 1u)) {
-        int pos = cnt + __popc(gmask & ((1u << jj) - 1u));
-        unsigned lm = (unsigned)((mb >> (jj * 8)) & 0xFFull);
-        unsigned codes = spread8((unsigned)((lob >> (jj * 8)) & 0xFFull)) |
-                         (spread8((unsigned)((hib >> (jj * 8)) & 0xFFull)) << 1);
-        if (pos < A.stride)
-          E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | (codes << 8) | (rank << 24));
+        for (int q = 0; q < kClusterSize; q++)
+          if (((rem >> q) & 1u) && (int)((imgs >> (5 * q)) & 31ull) == limg) m |= 1u << q;
+        rem &= ~m;
       }
-      cnt += __popc(gmask);
-      remaining &= ~mb;
+      const unsigned long long bb = __ballot(has);
+      const int pos = cnt + __popcll(bb & ((1ull << lane) - 1ull));
+      if (has && pos < A.stride) {
+        const unsigned sm = spread8(m);
+        E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), m | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
+      }
+      cnt += __popcll(bb);
     }
   }
   if (lane == 0) {
@@ -527,6 +684,7 @@ void Engine::build_list() {
   max_groups = ((max_clusters / kGroup + ncol + 1) + 7) / 8 * 8;
 
   wx.resize(n); wy.resize(n); wz.resize(n);
+  atom_wrap.resize(n);
   atom_col.resize(n);
   col_count.resize(ncol); col_fill.resize(ncol); col_atom_start.resize(ncol);
   col_cl_start.resize(ncol); col_ncl.resize(ncol);
@@ -548,7 +706,7 @@ void Engine::build_list() {
 
   hipLaunchKernelGGL(k_fill_int, dim3((ncol + T - 1) / T), dim3(T), 0, stream, col_count.p, 0, ncol);
   hipLaunchKernelGGL(k_bin, dim3(nb_atoms), dim3(T), 0, stream, n, g, x.p, y.p, z.p, wx.p, wy.p, wz.p,
-                     atom_col.p, col_count.p);
+                     atom_wrap.p, atom_col.p, col_count.p);
   hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, stream, ncol, col_count.p, col_atom_start.p,
                      col_cl_start.p, col_ncl.p, col_fill.p, col_grp_start.p, n_clusters_d.p, n_groups_d.p);
   hipLaunchKernelGGL(k_fill_groups, dim3((ncol + T - 1) / T), dim3(T), 0, stream, ncol, col_cl_start.p, col_ncl.p,
@@ -557,8 +715,12 @@ void Engine::build_list() {
                      col_fill.p, col_atoms.p);
   const int nslots = max_clusters * kClusterSize;
   hipLaunchKernelGGL(k_fill_int, dim3((nslots + T - 1) / T), dim3(T), 0, stream, slot_orig.p, -1, nslots);
-  hipLaunchKernelGGL(k_sort_columns, dim3(nb_atoms), dim3(T), 0, stream, n, atom_col.p, col_count.p,
-                     col_atom_start.p, col_cl_start.p, col_atoms.p, wz.p, slot_orig.p, orig_slot.p, cl_col.p);
+  hipLaunchKernelGGL(k_sort_columns, dim3(ncol), dim3(256), 0, stream, ncol, col_count.p, col_atom_start.p,
+                     col_cl_start.p, col_atoms.p, wz.p, slot_orig.p, orig_slot.p, cl_col.p);
+  slot_list.resize((size_t)nslots * sizeof(SlotList));
+  hipLaunchKernelGGL(k_slot_pack, dim3((nslots + T - 1) / T), dim3(T), 0, stream, nslots, n_clusters_d.p,
+                     slot_orig.p, wx.p, wy.p, wz.p, atom_wrap.p, sp_off.p,
+                     reinterpret_cast<SlotList *>(slot_list.p));
   hipLaunchKernelGGL(k_cluster_bbox, dim3((max_clusters + T - 1) / T), dim3(T), 0, stream, n_clusters_d.p,
                      max_clusters, slot_orig.p, wx.p, wy.p, wz.p, cl_lo.p, cl_hi.p);
 
@@ -577,6 +739,15 @@ void Engine::build_list() {
     A.half[d] = periodic ? box[d] / 2.0 : 0.0;
   }
   A.rlist2 = rlist * rlist;  // sqr(Cutoff + Padding), neighbourlist.cpp:355
+  // Wrapped positions differ from PD's by a few ulp of the box size, so their
+  // r^2 is within ~1e-11 A^2 of PD's; pairs within 1e-10 rlist^2 of the limit
+  // are decided by PD's arithmetic.
+  A.rlist2_lo = A.rlist2 * (1.0 - 1e-10);
+  A.rlist2_hi = A.rlist2 * (1.0 + 1e-10);
+  A.fast = 1;
+  if (periodic)  // inside rlist implies the minimum image only when rlist stays clear of half the box
+    for (int d = 0; d < 3; d++)
+      if (!(rlist * (1.0 + 1e-6) + 1e-6 < box[d] / 2.0)) A.fast = 0;
   A.Rbox2 = R * R;
   A.mid = mid;
   if (entry_capacity == 0) {
@@ -596,7 +767,8 @@ void Engine::build_list() {
       ScopedTimer t(*this, "pair_search");
       hipLaunchKernelGGL(k_pair_search, dim3(max_groups), dim3(64 * kGroup), 0, stream, A, n_groups_d.p,
                          grp_first.p, grp_count.p, cl_col.p, col_cl_start.p, col_ncl.p, cl_lo.p, cl_hi.p,
-                         slot_orig.p, x.p, y.p, z.p, wx.p, wy.p, wz.p, band.p, sp_off.p, sp_part.p, sp_ord.p,
+                         reinterpret_cast<const SlotList *>(slot_list.p), x.p, y.p, z.p, band.p, sp_off.p,
+                         sp_part.p, sp_ord.p,
                          entries.p, cl_entry_count.p, wg_ncand.p, wg_cand.p, entry_counter.p);
     }
     PDGPU_HIP(hipGetLastError());

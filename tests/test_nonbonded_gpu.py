@@ -20,8 +20,11 @@ def _engine(s, nb_kw, padding, shadow=False):
     return G.engine_for(s, G.nb_params(**nb_kw), padding=padding, calc_shadow=shadow)
 
 
-@pytest.mark.parametrize("n_waters,shadow", [(512, False), (729, True), (1000, False)])
+@pytest.mark.parametrize("n_waters,shadow", [(343, False), (512, False), (729, True), (1000, False)])
 def test_water_pair_set_exact(n_waters, shadow):
+    # 343 waters: a 21.7 A box, so Cutoff+Padding = 12 A exceeds half the box and
+    # every pair takes PD's minimum-image loops; the larger boxes are decided
+    # from the wrapped positions except near the limit.
     s = S.water_box(n_waters, seed=n_waters)
     print(f"case water n={n_waters} shadow={shadow}", flush=True)
     pos, counts, packed, _, _ = oracle_nb(s, WATER_NB, padding=2.0, shadow=shadow)
@@ -30,6 +33,52 @@ def test_water_pair_set_exact(n_waters, shadow):
     assert e.full_update_count() == 1
     gpos = e.get_positions()
     assert np.array_equal(gpos, pos), "cleanSpace positions differ"
+    gc, gp = e.get_pairs()
+    assert np.array_equal(gc, counts)
+    assert np.array_equal(gp, packed)
+
+
+def _band_pairs(rlist):
+    """A 512-water box (25 A) with positions on a 2^-20 A grid, and waters moved
+    so that chosen O-O pairs lie at rlist exactly or 2^-34 / 2^-40 A inside or
+    outside it, along x, directly and across the box face.  Every coordinate
+    stays exact in FP64, so PD's r^2 lands on the limit or one side of it."""
+    s = S.water_box(512, seed=77, box_len=25.0)
+    s.pos = np.round(s.pos * 2.0**20) / 2.0**20
+    L = s.box[0]
+    deltas = [(0.0, True), (-2.0**-34, True), (2.0**-34, False), (-2.0**-40, True), (2.0**-40, False)]
+    used, cases = set(), []
+    ox = s.pos[0::3, 0]
+    for cross in (False, True):
+        lo, hi = ((-11.0, -1.0) if not cross else (2.0, 11.0))
+        for d, expect in deltas:
+            a = next(w for w in range(len(ox)) if lo <= ox[w] <= hi and w not in used)
+            used.add(a)
+            b = next(w for w in range(len(ox)) if w not in used)
+            used.add(b)
+            ia, ib = 3 * a, 3 * b
+            target = s.pos[ia].copy()
+            target[0] += rlist + d - (L if cross else 0.0)
+            s.pos[ib:ib + 3] += target - s.pos[ib]
+            assert s.pos[ib, 0] - s.pos[ia, 0] == rlist + d - (L if cross else 0.0)
+            cases.append((ia, ib, expect))
+    return s, cases
+
+
+def _listed(counts, packed, a, b):
+    start = np.concatenate([[0], np.cumsum(counts)])
+    hi, lo = max(a, b), min(a, b)
+    return lo in (packed[start[hi]:start[hi + 1]] & 0xFFFFFF)
+
+
+def test_water_pair_set_exact_at_the_limit():
+    s, cases = _band_pairs(12.0)
+    print("case water band pairs", flush=True)
+    pos, counts, packed, _, _ = oracle_nb(s, WATER_NB, padding=2.0)
+    for a, b, expect in cases:  # the construction really puts pairs on both sides of the limit
+        assert _listed(counts, packed, a, b) == expect, (a, b)
+    e = _engine(s, WATER_NB, 2.0)
+    e.build_list()
     gc, gp = e.get_pairs()
     assert np.array_equal(gc, counts)
     assert np.array_equal(gp, packed)
