@@ -161,6 +161,13 @@ struct GBSys {
 };
 
 struct Engine;
+// The per-step trace entry of pdgpu_md, written after the step's energy sums.
+struct StepRecord {
+  double *trace;
+  int step;
+  int use_nb, use_bonded, use_gb, use_sasa;
+  double vdwcor, kcal_to_J;
+};
 
 // Device pointers of FF_Bonded terms (k_bonded.hip); force constants in kcal/mol units.
 struct BondedArrays {
@@ -294,7 +301,8 @@ struct Engine {
   DBuf<uint2> pentries;
   DBuf<int> pcount;
   DBuf<SlotPos> pref;
-  DBuf<unsigned> prune_dmax;               // max displacement from pref this step (float bits)
+  DBuf<unsigned> prune_dmax;               // [2]: max displacement from pref (float bits), alternating per step
+  int dmax_cur = 0;                        // which prune_dmax word this evaluation fills
   double prune_margin = 0.5;
   bool prune_forced = true;
   DBuf<unsigned long long> entry_counter;  // [0] entries used, [1] overflow flag
@@ -307,6 +315,16 @@ struct Engine {
   DBuf<double> e_partial_ke;               // per-block kinetic energy partials (4 per block)
   DBuf<double> e_out;                      // reduced energies, 4 per component (see Slots)
   DBuf<double> trace;                      // per-step epot / ekin of pdgpu_md
+  // energy partials (4 per block) summed by one kernel at the end of an
+  // evaluation, or of an MD step together with its trace entry
+  struct PendingSum {
+    const double *part;
+    int count;
+    double *out;
+  };
+  std::vector<PendingSum> pending_sums;
+  void defer_sum4(const double *part, int count, double *out) { pending_sums.push_back({part, count, out}); }
+  void flush_sums(const StepRecord *rec = nullptr);
 
   // ---- bonded terms -------------------------------------------------------------
   int nbonds = 0, nangles = 0, ntors = 0;
@@ -339,14 +357,15 @@ struct Engine {
   // nonbonded.hip
   void prepare_slots();
   void setup_nonbonded(const pdgpu_nb_params &p);
-  void gather_positions();
   void launch_nonbonded(double *energy_slots, bool forces = true);  // energies: vdw, elec, virial
   // bonded.hip
   void launch_bonded(double *energy_slots);      // energies: bond, angle, torsion, virial
   // integrate.hip
-  void zero_forces();
+  void begin_evaluation(bool nonbonded);  // clears accumulators and energy slots (+ gathers slot positions)
   void forces_to_host(double *f, size_t stride_bytes);
-  void evaluate(uint32_t components, int step = -1);  // forces + energies into e_out (step < 0: LCPO in full)
+  // forces + energies into e_out (step < 0: LCPO in full); flush = false leaves the
+  // energy sums pending for the caller's flush_sums
+  void evaluate(uint32_t components, int step = -1, bool flush = true);
   void energies_to_host(uint32_t components, pdgpu_energies *out);
   void integrate(const pdgpu_md_params &p, int step, double *ekin_slot);
   void md(const pdgpu_md_params &p, int n_steps, double *epot_trace, double *ekin_trace);
@@ -406,7 +425,7 @@ struct ScopedTimerOpt {
 };
 
 void langevin_coefficients(double gammat, double *c6);
-void reduce4(hipStream_t s, const double *part, int count, double *out);
+
 
 // ---- device helpers ------------------------------------------------------------
 

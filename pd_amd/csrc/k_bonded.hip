@@ -273,7 +273,7 @@ void Engine::launch_bonded(double *energy_out) {
                  t_i.p, t_a.p, t_b.p, t_j.p, t_terms.p, t_vn.p, t_nn.p, t_gamma.p, b_win.p};
   launch_bonded_kernel(stream, B, x.p, y.p, z.p, fbx.p, fby.p, fbz.p, part.p, nullptr);
   PDGPU_HIP(hipGetLastError());
-  reduce4(stream, part.p, grid, energy_out);
+  defer_sum4(part.p, grid, energy_out);
 }
 
 }  // namespace pdgpu

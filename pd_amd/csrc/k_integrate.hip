@@ -15,29 +15,63 @@ namespace pdgpu {
 
 namespace {
 
-__global__ void k_zero3(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c,
-                        int count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) {
-    a[i] = 0;
-    b[i] = 0;
-    c[i] = 0;
-  }
-}
+// Sums of 4-wide energy partials, one list entry after the other, by one
+// workgroup in a fixed order; then, for an MD step, the step's trace entry:
+// trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
+// per component slot; the kinetic energy slot is in J).
+constexpr int kMaxSums = 8;
+struct SumList {
+  const double *part[kMaxSums];
+  double *out[kMaxSums];
+  int count[kMaxSums];
+  int n;
+  int record;
+  StepRecord rec;
+  const double *e_out;
+};
 
-__global__ void k_reduce4_kernel(const double *__restrict__ part, int count, double *__restrict__ out) {
-  __shared__ double s[4][256];
-  double a[4] = {0, 0, 0, 0};
-  for (int i = threadIdx.x; i < count; i += 256)
-    for (int d = 0; d < 4; d++) a[d] += part[4 * i + d];
-  for (int d = 0; d < 4; d++) s[d][threadIdx.x] = a[d];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int d = 0; d < 4; d++) s[d][threadIdx.x] += s[d][threadIdx.x + w];
+__global__ __launch_bounds__(1024) void k_sums(SumList L) {
+  __shared__ double s[16][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int e = 0; e < L.n; e++) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
+    for (int i = threadIdx.x; i < L.count[e]; i += 1024) {
+      const double4 v = P[i];
+      a0 += v.x;
+      a1 += v.y;
+      a2 += v.z;
+      a3 += v.w;
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_sum(a3);
+    if (lane == 0) {
+      s[w][0] = a0;
+      s[w][1] = a1;
+      s[w][2] = a2;
+      s[w][3] = a3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double t = 0.0;
+      for (int k = 0; k < 16; k++) t += s[k][threadIdx.x];
+      L.out[e][threadIdx.x] = t;
+    }
     __syncthreads();
   }
-  if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
+  if (L.record && threadIdx.x == 0) {
+    const double *E = L.e_out;
+    const StepRecord &R = L.rec;
+    double ep = 0.0;
+    if (R.use_sasa) ep += E[13];
+    if (R.use_nb) ep += E[0] + E[1] + R.vdwcor;
+    if (R.use_bonded) ep += E[4] + E[5] + E[6];
+    if (R.use_gb) ep += E[8] + E[9] + E[10] + E[12];
+    R.trace[2 * R.step] = ep * R.kcal_to_J;
+    R.trace[2 * R.step + 1] = E[16];
+  }
 }
 
 // Total force of atom i in J/m: the original-order accumulator plus the
@@ -217,49 +251,43 @@ __global__ __launch_bounds__(256) void k_langevin(
   block_sum4(ke, ke_partial);
 }
 
-// trace[2*s] = epot (J), trace[2*s+1] = ekin (J); energies are in kcal/mol
-// in e_out (4 doubles per component slot).
-__global__ void k_record(const double *__restrict__ e_out, int use_nb, int use_bonded, int use_gb, int use_sasa,
-                         double vdwcor, double kcal_to_J, double *__restrict__ trace, int s) {
-  if (threadIdx.x != 0) return;
-  double ep = 0.0;
-  if (use_sasa) ep += e_out[13];
-  if (use_nb) ep += e_out[0] + e_out[1] + vdwcor;
-  if (use_bonded) ep += e_out[4] + e_out[5] + e_out[6];
-  if (use_gb) ep += e_out[8] + e_out[9] + e_out[10] + e_out[12];
-  trace[2 * s] = ep * kcal_to_J;
-  trace[2 * s + 1] = e_out[16];
-}
-
 }  // namespace
 
-void reduce4(hipStream_t s, const double *part, int count, double *out) {
-  hipLaunchKernelGGL(k_reduce4_kernel, dim3(1), dim3(256), 0, s, part, count, out);
+void Engine::flush_sums(const StepRecord *rec) {
+  if (pending_sums.empty() && !rec) return;
+  size_t k = 0;
+  do {  // more than kMaxSums entries: several launches, in order
+    SumList L{};
+    for (; k < pending_sums.size() && L.n < kMaxSums; k++, L.n++) {
+      L.part[L.n] = pending_sums[k].part;
+      L.out[L.n] = pending_sums[k].out;
+      L.count[L.n] = pending_sums[k].count;
+    }
+    const bool last = k == pending_sums.size();
+    L.record = (last && rec) ? 1 : 0;
+    if (rec) L.rec = *rec;
+    L.e_out = e_out.p;
+    hipLaunchKernelGGL(k_sums, dim3(1), dim3(1024), 0, stream, L);
+  } while (k < pending_sums.size());
+  pending_sums.clear();
+  PDGPU_HIP(hipGetLastError());
 }
 
 // e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
 // bond, angle, torsion, virial; [8..11] GB pol, vdw, elec, virial (see k_gb.hip);
 // [12] GB self; [13] LCPO cavity energy; [16] kinetic energy (J); others spare.
-void Engine::zero_forces() {
-  fbx.resize(n);
-  fby.resize(n);
-  fbz.resize(n);
-  hipLaunchKernelGGL(k_zero3, dim3((n + 255) / 256), dim3(256), 0, stream, fbx.p, fby.p, fbz.p, n);
-}
-
-void Engine::evaluate(uint32_t components, int step) {
+void Engine::evaluate(uint32_t components, int step, bool flush) {
   const bool eonly = (components & PDGPU_ENERGY_ONLY) != 0;
-  e_out.resize(24);
-  PDGPU_HIP(hipMemsetAsync(e_out.p, 0, 24 * sizeof(double), stream));
-  zero_forces();
-  if (components & PDGPU_NONBONDED) {
-    if (!list_valid) build_list();
-    launch_nonbonded(e_out.p + 0, !eonly);
-  }
+  const bool nbon = (components & PDGPU_NONBONDED) != 0;
+  pending_sums.clear();
+  if (nbon && !list_valid) build_list();
+  begin_evaluation(nbon);
+  if (nbon) launch_nonbonded(e_out.p + 0, !eonly);
   if ((components & PDGPU_BONDED) && (nbonds + nangles + ntors) > 0) launch_bonded(e_out.p + 4);
   if (components & PDGPU_GB) launch_gb(e_out.p + 8, eonly);
   if (components & PDGPU_SASA) launch_lcpo(e_out.p + 13, step, step < 0, eonly);
   last_components = components;
+  if (flush) flush_sums();
 }
 
 void Engine::forces_to_host(double *f, size_t stride_bytes) {
@@ -355,7 +383,7 @@ void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
                        orig_slot.p, x.p, y.p, z.p, vx.p, vy.p, vz.p, e_partial_ke.p);
   }
   PDGPU_HIP(hipGetLastError());
-  reduce4(stream, e_partial_ke.p, grid, ke_slot);
+  defer_sum4(e_partial_ke.p, grid, ke_slot);
 }
 
 // c0 c1 c2 varr varv crv of applyForces_LangevinIntegration (md.cpp:947-986).
@@ -406,11 +434,11 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
   for (int s = 0; s < n_steps; s++) {
     const int step = p.step_offset + s;
     if ((comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0) build_list();
-    evaluate(comp, step);
+    evaluate(comp, step, false);
     integrate(p, step, e_out.p + 16);
-    hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, stream, e_out.p, (comp & PDGPU_NONBONDED) ? 1 : 0,
-                       (comp & PDGPU_BONDED) ? 1 : 0, (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor,
-                       Consts::kcal_to_J(), trace.p, s);
+    const StepRecord rec{trace.p, s, (comp & PDGPU_NONBONDED) ? 1 : 0, (comp & PDGPU_BONDED) ? 1 : 0,
+                         (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor, Consts::kcal_to_J()};
+    flush_sums(&rec);
   }
   std::vector<double> h((size_t)n_steps * 2);
   trace.download(h.data(), h.size(), stream);

@@ -387,8 +387,6 @@ void LCPOState::run(hipStream_t s, const double *x, const double *y, const doubl
 
 namespace pdgpu {
 
-void reduce4(hipStream_t s, const double *part, int count, double *out);
-
 namespace {
 // Sums per-atom energies into per-block partials (4 per block, first slot used).
 __global__ void k_block_sum(int n, const double *__restrict__ v, double *__restrict__ part) {
@@ -413,7 +411,7 @@ void Engine::launch_lcpo(double *energy_slot, int step, bool full, bool energy_o
   const int blocks = (n + 255) / 256;
   e_partial_lcpo.resize((size_t)blocks * 4);
   hipLaunchKernelGGL(k_block_sum, dim3(blocks), dim3(256), 0, stream, n, lcpo->eatom.p, e_partial_lcpo.p);
-  reduce4(stream, e_partial_lcpo.p, blocks, energy_slot);
+  defer_sum4(e_partial_lcpo.p, blocks, energy_slot);
 }
 
 }  // namespace pdgpu

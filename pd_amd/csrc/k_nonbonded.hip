@@ -46,30 +46,52 @@ __device__ __forceinline__ double image_shift(int img, int axis, double L) {
   return k == 1 ? 0.0 : (k == 2 ? L : -L);
 }
 
-__global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
-                               const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ z, SlotPos *__restrict__ spos,
-                               const SlotPos *__restrict__ pref, unsigned *__restrict__ dmax) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots || s >= *n_clusters * kClusterSize) return;
-  int a = slot_orig[s];
-  SlotPos p;
-  if (a >= 0) {
-    p.x = x[a];
-    p.y = y[a];
-    p.z = z[a];
-    // displacement since the last prune, rounded up (float bits order like the values)
-    const SlotPos r = pref[s];
-    const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-    const float d = (float)(sqrt(dx * dx + dy * dy + dz * dz) * (1.0 + 1e-6));
-    atomicMax(dmax, __float_as_uint(d == d ? d : 3.0e38f));
-  } else {  // padding slot: far away, zero parameters, never listed
-    p.x = 1.0e6;
-    p.y = 1.0e6;
-    p.z = 1.0e6;
+// First kernel of an evaluation with the nonbonded term.  Gathers the FP64
+// positions into slot order, folds each atom's displacement since the last
+// prune into dmax[cur] (one atomic per wave; dmax[1 - cur] is cleared for the
+// next evaluation) and clears the force accumulators and energy slots.
+__global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
+                           const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                           SlotPos *__restrict__ spos, const SlotPos *__restrict__ pref, unsigned *__restrict__ dmax,
+                           int cur, long long *__restrict__ nbx, long long *__restrict__ nby,
+                           long long *__restrict__ nbz, long long *__restrict__ fbx, long long *__restrict__ fby,
+                           long long *__restrict__ fbz, double *__restrict__ e_out, int n_e) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0 && dmax) dmax[1 - cur] = 0u;
+  if (s < n_e) e_out[s] = 0.0;
+  if (s < n) {
+    fbx[s] = 0;
+    fby[s] = 0;
+    fbz[s] = 0;
   }
-  p.pad = 0.0;
-  spos[s] = p;
+  double d = 0.0;
+  if (s < nslots) {
+    nbx[s] = 0;
+    nby[s] = 0;
+    nbz[s] = 0;
+    if (s < *n_clusters * kClusterSize) {
+      const int a = slot_orig[s];
+      SlotPos p;
+      if (a >= 0) {
+        p.x = x[a];
+        p.y = y[a];
+        p.z = z[a];
+        const SlotPos r = pref[s];
+        const double dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(d == d)) d = 3.0e38;  // NaN positions force a prune
+      } else {  // padding slot: far away, zero parameters, never listed
+        p.x = 1.0e6;
+        p.y = 1.0e6;
+        p.z = 1.0e6;
+      }
+      p.pad = 0.0;
+      spos[s] = p;
+    }
+  }
+  d = wave_max(d);
+  // rounded up (float bits of non-negative values order like the values)
+  if ((threadIdx.x & 63) == 0 && d > 0.0) atomicMax(&dmax[cur], __float_as_uint((float)(d * (1.0 + 1e-6))));
 }
 
 // Rolling prune of the list: keeps the pairs within Cutoff + margin at the
@@ -80,7 +102,7 @@ __global__ void k_gather_slots(int nslots, const int *__restrict__ n_clusters, c
 // their order.
 __global__ __launch_bounds__(64) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
                                               float rk2, float margin_ok, int forced,
-                                              const unsigned *__restrict__ dmax,
+                                              const unsigned *__restrict__ dmax,  // this evaluation's word
                                               const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
                                               const int *__restrict__ cl_count, int stride,
                                               uint2 *__restrict__ pentries, int *__restrict__ pcount,
@@ -137,17 +159,6 @@ __global__ void k_gather_params(int nslots, const int *__restrict__ n_clusters, 
   if (s >= nslots || s >= *n_clusters * kClusterSize) return;
   int a = slot_orig[s];
   sprm[s] = a >= 0 ? make_float4(q[a], r[a], e[a], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
-__global__ void k_zero_ll(long long *__restrict__ a, long long *__restrict__ b, long long *__restrict__ c,
-                          int count, unsigned *__restrict__ dmax) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *dmax = 0u;
-  if (i < count) {
-    a[i] = 0;
-    b[i] = 0;
-    c[i] = 0;
-  }
 }
 
 __device__ __forceinline__ void lds_fix_add(long long *p, long long v) {
@@ -387,8 +398,6 @@ void launch_mode(Engine &E, bool vdw, bool forces) {
 
 }  // namespace
 
-void reduce4(hipStream_t s, const double *part, int count, double *out);
-
 void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
   require(!h_charge.empty(), "atom parameters not set");
   // FF_NonBonded::setupBasisVectors refuses more than 32 image vectors (nonbonded.cpp:1059-1069);
@@ -471,13 +480,31 @@ void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
   list_valid = false;
 }
 
-void Engine::gather_positions() {
+void Engine::begin_evaluation(bool nonbonded) {
+  const int T = 256;
+  fbx.resize(n);
+  fby.resize(n);
+  fbz.resize(n);
+  e_out.resize(24);
+  if (!nonbonded) {
+    hipLaunchKernelGGL(k_begin_nb, dim3((std::max(n, 24) + T - 1) / T), dim3(T), 0, stream, 0, n, n_clusters_d.p,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                       fbx.p, fby.p, fbz.p, e_out.p, 24);
+    return;
+  }
+  require(list_valid, "neighbour list not built");
   const int nslots = max_clusters * kClusterSize;
   spos.resize(nslots);
   pref.resize(nslots);
-  prune_dmax.resize(1);
-  hipLaunchKernelGGL(k_gather_slots, dim3((nslots + 255) / 256), dim3(256), 0, stream, nslots, n_clusters_d.p,
-                     slot_orig.p, x.p, y.p, z.p, spos.p, pref.p, prune_dmax.p);
+  if (prune_dmax.n < 2) {
+    prune_dmax.resize(2);
+    PDGPU_HIP(hipMemsetAsync(prune_dmax.p, 0, 2 * sizeof(unsigned), stream));
+  }
+  dmax_cur ^= 1;
+  const int m = std::max(std::max(nslots, n), 24);
+  hipLaunchKernelGGL(k_begin_nb, dim3((m + T - 1) / T), dim3(T), 0, stream, nslots, n, n_clusters_d.p, slot_orig.p,
+                     x.p, y.p, z.p, spos.p, pref.p, prune_dmax.p, dmax_cur, nbx.p, nby.p, nbz.p, fbx.p, fby.p, fbz.p,
+                     e_out.p, 24);
 }
 
 // Per-slot parameters and accumulators; called once per list rebuild.
@@ -495,11 +522,6 @@ void Engine::prepare_slots() {
 void Engine::launch_nonbonded(double *energy_out, bool forces) {
   require(have_nb, "nonbonded component not set up");
   require(list_valid, "neighbour list not built");
-  const int nslots = max_clusters * kClusterSize;
-  prune_dmax.resize(1);
-  hipLaunchKernelGGL(k_zero_ll, dim3((nslots + 255) / 256), dim3(256), 0, stream, nbx.p, nby.p, nbz.p, nslots,
-                     prune_dmax.p);
-  gather_positions();
   {
     ScopedTimer t(*this, "prune");
     pentries.resize((size_t)max_clusters * entry_capacity);
@@ -507,7 +529,7 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
     hipLaunchKernelGGL(k_prune, dim3(max_clusters), dim3(64), 0, stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
-                       (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p, spos.p,
+                       (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur, spos.p,
                        entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
     prune_forced = false;
   }
@@ -524,7 +546,7 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     }
   }
   PDGPU_HIP(hipGetLastError());
-  reduce4(stream, e_partial.p, waves, energy_out);
+  defer_sum4(e_partial.p, waves, energy_out);
 }
 
 }  // namespace pdgpu
