@@ -218,7 +218,6 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
 #pragma unroll
     for (int a = 0; a < 8; a++) fix[a] = fiy[a] = fiz[a] = 0.0f;
     const float cut2f = (float)P.cut2, vcut2f = (float)P.vdwcut2;
-    const bool split_vdw = P.vdwcut2 != P.cut2;
 
     const int cnt = cl_count[ci];
     const uint2 *E = entries + (size_t)ci * stride;
@@ -245,21 +244,41 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
       // registers: the empty asm hides that the index is loop-invariant.
       int ib = wave * kClusterSize;
       asm volatile("" : "+v"(ib));
+      // cutoff decisions for the 8 pairs first: FP32 r^2 from the FP64 relative
+      // displacement; a pair within kNearBand of a cutoff takes PD's FP64 test
+      // (nonbonded.cpp:718-722) behind one wave-uniform branch, rarely taken
+      float dxa[8], dya[8], dza[8], r2a[8];
+      bool inca[8], inva[8];
+      bool anynear = false;
 #pragma unroll
       for (int a = 0; a < 8; a++) {
         const double *io = &ioff[0][0][0] + 4 * (ib + a);
-        const float dx = (float)(ax - io[0]), dy = (float)(ay - io[1]), dz = (float)(az - io[2]);
-        const float r2s_a = dx * dx + dy * dy + dz * dz;
-        bool inc = r2s_a <= cut2f, inv = r2s_a < vcut2f;
-        const bool near = fabsf(r2s_a - cut2f) < kNearBand || (split_vdw && fabsf(r2s_a - vcut2f) < kNearBand);
-        if (near && ((lm >> a) & 1u)) {  // rare: PD's FP64 test, nonbonded.cpp:718-722
+        dxa[a] = (float)(ax - io[0]);
+        dya[a] = (float)(ay - io[1]);
+        dza[a] = (float)(az - io[2]);
+        r2a[a] = dxa[a] * dxa[a] + dya[a] * dya[a] + dza[a] * dza[a];
+        inca[a] = r2a[a] <= cut2f;
+        inva[a] = r2a[a] < vcut2f;
+        anynear |= ((lm >> a) & 1u) && fminf(fabsf(r2a[a] - cut2f), fabsf(r2a[a] - vcut2f)) < kNearBand;
+      }
+      if (__builtin_expect(__ballot(anynear) != 0ull, 0)) {
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const bool near = ((lm >> a) & 1u) && fminf(fabsf(r2a[a] - cut2f), fabsf(r2a[a] - vcut2f)) < kNearBand;
+          if (near) {
 #pragma clang fp contract(off)
-          const SlotPos pi = spos[ci * kClusterSize + a];
-          const double ex = (pj.x - pi.x) + shx, ey = (pj.y - pi.y) + shy, ez = (pj.z - pi.z) + shz;
-          const double r2 = ex * ex + ey * ey + ez * ez;
-          inc = r2 <= P.cut2;
-          inv = r2 < P.vdwcut2;
+            const SlotPos pi = spos[ci * kClusterSize + a];
+            const double ex = (pj.x - pi.x) + shx, ey = (pj.y - pi.y) + shy, ez = (pj.z - pi.z) + shz;
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            inca[a] = r2 <= P.cut2;
+            inva[a] = r2 < P.vdwcut2;
+          }
         }
+      }
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        const float dx = dxa[a], dy = dya[a], dz = dza[a], r2s_a = r2a[a];
+        const bool inc = inca[a], inv = inva[a];
         const float4 pa = (&iprm[0][0])[ib + a];
         const bool on = ((lm >> a) & 1u) && !((excl >> (2 * a)) & 1u) && inc;
         const bool s14 = (is14 >> (2 * a)) & 1u;
@@ -368,8 +387,8 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 
   __syncthreads();
   // flush the touched slots of the candidate table, consecutive threads on
   // consecutive slots of a j-cluster
-  const int ns = FORCES ? nr * kClusterSize : 0;
-  for (int t = threadIdx.x; t < 3 * ns; t += blockDim.x) {
+  const int ns = nr * kClusterSize;
+  for (int t = threadIdx.x; FORCES && t < 3 * ns; t += blockDim.x) {
     const int c = t / ns, l = t - c * ns;
     const long long v = jacc[c][l];
     if (v != 0) {
