@@ -60,9 +60,33 @@ def pair_classes(eng, box, cutoff, inner):
     return n_tot, in_cut, inner_cnt
 
 
-def cpu_baseline(sysw, nb_kw, padding):
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "water_box_98304_hbm_bytes.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE runs of this bench, each scaled by what the counter
+    reads for a copy of known size); {} when the summary is absent."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("PMC_JSON "):
+                d = json.loads(line[len("PMC_JSON "):])
+                if kernel in d:
+                    return {"bytes_per_launch": d[kernel]["hbm_bytes_per_launch"],
+                            "source": os.path.relpath(PMC_SUMMARY, os.path.dirname(PMC_SUMMARY) + "/..")}
+    except OSError:
+        pass
+    return {}
+
+
+def cpu_baseline(sysw, nb_kw, padding, eng=None, comps=0):
     """The FP64 oracle (PD's algorithms restated in C, one thread) on the same box:
-    one O(N^2) list build, one nonbonded + bonded evaluation and one VV step."""
+    one O(N^2) list build, one nonbonded + bonded evaluation and one VV step.
+
+    With `eng`, the engine evaluates the same positions as well, and the result
+    carries the error of the device forces and energy against the oracle: the
+    error the headline number is stated at."""
     from oracle import oracle as O
     t0 = time.perf_counter()
     pos = O.clean_space(sysw)
@@ -73,22 +97,35 @@ def cpu_baseline(sysw, nb_kw, padding):
     t_list = time.perf_counter() - t0
     basis = O.basis_vectors(True, sysw.box, cut)
     t1 = time.perf_counter()
-    f, _ = O.nb_forces(sysw, pos, counts, packed, p, basis)
-    fb, _ = O.bonded_forces(sysw, pos)
+    f, e_nb = O.nb_forces(sysw, pos, counts, packed, p, basis)
+    fb, e_b = O.bonded_forces(sysw, pos)
     vel = np.zeros_like(pos)
-    O.vv_step(sysw.mass, 1e-15, f + fb, pos, vel)
+    O.vv_step(sysw.mass, 1e-15, f + fb, pos.copy(), vel)
     t_step = time.perf_counter() - t1
     per_step = t_list / 10.0 + t_step
-    return {
+    out = {
         "value": round(86400.0 / per_step * 1e-6, 6),  # 1 fs per step -> ns/day
         "unit": "ns/day",
         "cores": 1,
         "kind": "port",
         "sample": (f"oracle (PD's O(N^2) list build + fast2 nonbonded + bonded + velocity Verlet, "
-                   f"FP64, 1 thread) on the same 98,304-atom box: 1 list build {t_list:.2f} s "
-                   f"(amortised over 10 steps) + 1 force/integrator step {t_step:.2f} s"),
+                   f"FP64, 1 thread) on the same 98,304-atom box at the end of the timed run: 1 list "
+                   f"build {t_list:.2f} s (amortised over 10 steps) + 1 force/integrator step {t_step:.2f} s"),
         "s_per_step": per_step,
     }
+    if eng is not None:
+        eng.set_positions(pos)
+        eng.build_list()
+        gf, ge = eng.compute(comps)
+        fo = f + fb
+        eo = e_nb["epot"] + e_b["epot"]
+        out["error"] = {
+            "force_rms_rel": float(np.sqrt(((gf - fo) ** 2).mean()) / np.sqrt((fo ** 2).mean())),
+            "energy_rel": float(abs(ge["epot"] - eo) / abs(eo)),
+            "epot_kcal_mol": round(eo * 6.02214076e23 / 4184.0, 3),
+            "reference": "FP64 oracle at the same positions (nonbonded + bonded)",
+        }
+    return out
 
 
 GB_KW = dict(Cutoff=999.0, InnerCutoff=999.0, VdwCutoff=999.0, VdwInnerCutoff=999.0, GbsaStillCutoff=999.0,
@@ -321,6 +358,7 @@ def main():
     etot = (np.asarray(ep) + np.asarray(ek)) * U.J_TO_KCALMOL
     drift = float(np.polyfit(np.arange(len(etot)), etot, 1)[0]) * 1e6 if len(etot) > 2 else 0.0  # per ns at 1 fs
 
+    nb_traffic = pmc_traffic("k_nonbonded")
     out = {
         "metric": "ns/day @98k-atom water box; batched GB/SA steps/s vs plain, at stated error",
         "value": round(ns_day, 3),
@@ -352,7 +390,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": nb_traffic.get("bytes_per_launch"),
+            "traffic_source": nb_traffic.get("source"),
             "avg_launch_us": round(nb_avg_s * 1e6, 3),
             "launches": nb_launch,
             "algorithmic_flop_per_launch": int(flops),
@@ -368,13 +407,18 @@ def main():
             "prune_launches": pr_launch,
         },
     }
-    if not args.no_gb:
-        del eng
-        out["gbsa"] = gb_benchmarks(args, log)
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline (oracle, 1 thread)...")
-        cb = cpu_baseline(w, nb_kw, padding)
+        log("timing the CPU baseline (oracle, 1 thread) and the error against it...")
+        w_end = w.copy()
+        w_end.pos = eng.get_positions()
+        cb = cpu_baseline(w_end, nb_kw, padding, eng, comps)
+        out["error"] = cb["error"]
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        log(f"CPU baseline {cb['value']} ns/day; force rms rel {cb['error']['force_rms_rel']:.2e}, "
+            f"energy rel {cb['error']['energy_rel']:.2e}")
+    del eng
+    if not args.no_gb:
+        out["gbsa"] = gb_benchmarks(args, log)
     print(json.dumps(out), flush=True)
 
 
