@@ -38,9 +38,11 @@ constexpr int kClusterSize = 8;
 // one wavefront each.  Their list shares one table of up to kGroupRanks
 // candidate j-clusters, which the nonbonded kernel mirrors in LDS to sum the j
 // forces before touching global memory (rank kGroupRanks = no LDS slot).
+// 127 ranks keep the table at 24 KB of LDS, so four work groups share a CU; the
+// rarer j-clusters beyond it add straight to global memory.
 constexpr int kGroup = 4;
-constexpr int kGroupRanks = 255;
-constexpr int kGroupStride = 256;
+constexpr int kGroupRanks = 127;
+constexpr int kGroupStride = 128;
 constexpr int kWave = 64;
 
 // PD constants (forcefield.cpp:35-55), FP64.

@@ -166,7 +166,7 @@ __device__ __forceinline__ void lds_fix_add(long long *p, long long v) {
 }
 
 template <int ELEC, bool VDW, bool FORCES>
-__global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_nonbonded(
+__global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nonbonded(
     NBKernelParams P, const int *__restrict__ n_groups, const int *__restrict__ grp_first,
     const int *__restrict__ grp_count, const int *__restrict__ wg_ncand, const int *__restrict__ wg_cand,
     double Lx, double Ly, double Lz, const SlotPos *__restrict__ spos, const float4 *__restrict__ sprm,
