@@ -100,54 +100,75 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
 // rounding allowance) the pruned list holds every pair within the cutoff; the
 // kernel returns at once in that case.  One wave per i-cluster, entries kept in
 // their order.
-__global__ __launch_bounds__(64) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
-                                              float rk2, float margin_ok, int forced,
-                                              const unsigned *__restrict__ dmax,  // this evaluation's word
-                                              const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
-                                              const int *__restrict__ cl_count, int stride,
-                                              uint2 *__restrict__ pentries, int *__restrict__ pcount,
-                                              SlotPos *__restrict__ pref) {
+// Four i-clusters per workgroup, one wave each; a wave takes 256 entries per
+// round (four per lane) with every load of the round issued before the tests.
+constexpr int kPruneDepth = 4;
+__global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
+                                               float rk2, float margin_ok, int forced,
+                                               const unsigned *__restrict__ dmax,  // this evaluation's word
+                                               const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
+                                               const int *__restrict__ cl_count, int stride,
+                                               uint2 *__restrict__ pentries, int *__restrict__ pcount,
+                                               SlotPos *__restrict__ pref) {
+  __shared__ double shtab[27 * 3];
   if (!forced && 2.0f * __uint_as_float(*dmax) <= margin_ok) return;
-  const int ci = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (ci >= *n_clusters) return;
+  if (threadIdx.x < 81) {
+    const int axis = threadIdx.x % 3;
+    shtab[threadIdx.x] = image_shift(threadIdx.x / 3, axis, axis == 0 ? Lx : (axis == 1 ? Ly : Lz));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (ci >= *n_clusters) return;  // whole waves; no barrier follows
   if (lane < kClusterSize) pref[ci * kClusterSize + lane] = spos[ci * kClusterSize + lane];
+  // FP32 offsets from the cluster's first atom: their rounding (~1e-6 A) is far
+  // inside the 1e-3 A allowance of the prune criterion
   const SlotPos oi = spos[ci * kClusterSize];
-  double iox[8], ioy[8], ioz[8];
+  float iox[8], ioy[8], ioz[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) {
     const SlotPos p = spos[ci * kClusterSize + a];
-    iox[a] = p.x - oi.x;
-    ioy[a] = p.y - oi.y;
-    ioz[a] = p.z - oi.z;
+    iox[a] = (float)(p.x - oi.x);
+    ioy[a] = (float)(p.y - oi.y);
+    ioz[a] = (float)(p.z - oi.z);
   }
   const int cnt = cl_count[ci];
   const uint2 *E = entries + (size_t)ci * stride;
   uint2 *PE = pentries + (size_t)ci * stride;
   int out = 0;
-  for (int k0 = 0; k0 < cnt; k0 += 64) {
-    const int k = k0 + lane;
-    uint2 ent = k < cnt ? E[k] : make_uint2(0u, 0u);
-    unsigned keep = 0u;
-    if (k < cnt) {
-      const int j = (int)(ent.x & 0xFFFFFFu);
-      const int img = (int)((ent.x >> 24) & 31u);
-      const SlotPos pj = spos[j];
-      const double ax = (pj.x - oi.x) + image_shift(img, 0, Lx), ay = (pj.y - oi.y) + image_shift(img, 1, Ly),
-                   az = (pj.z - oi.z) + image_shift(img, 2, Lz);
+  for (int k0 = 0; k0 < cnt; k0 += 64 * kPruneDepth) {
+    uint2 ent[kPruneDepth];
+    SlotPos pj[kPruneDepth];
 #pragma unroll
-      for (int a = 0; a < 8; a++) {
-        const float dx = (float)(ax - iox[a]), dy = (float)(ay - ioy[a]), dz = (float)(az - ioz[a]);
-        keep |= ((dx * dx + dy * dy + dz * dz) <= rk2 ? 1u : 0u) << a;
+    for (int u = 0; u < kPruneDepth; u++) {
+      const int k = k0 + 64 * u + lane;
+      ent[u] = k < cnt ? E[k] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kPruneDepth; u++) {
+      const int k = k0 + 64 * u + lane;
+      if (k < cnt) pj[u] = spos[ent[u].x & 0xFFFFFFu];
+    }
+#pragma unroll
+    for (int u = 0; u < kPruneDepth; u++) {
+      const int k = k0 + 64 * u + lane;
+      unsigned keep = 0u;
+      if (k < cnt) {
+        const int img = (int)((ent[u].x >> 24) & 31u);
+        const float ax = (float)((pj[u].x - oi.x) + shtab[3 * img]),
+                    ay = (float)((pj[u].y - oi.y) + shtab[3 * img + 1]),
+                    az = (float)((pj[u].z - oi.z) + shtab[3 * img + 2]);
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const float dx = ax - iox[a], dy = ay - ioy[a], dz = az - ioz[a];
+          keep |= ((dx * dx + dy * dy + dz * dz) <= rk2 ? 1u : 0u) << a;
+        }
+        keep &= ent[u].y & 0xFFu;
       }
-      keep &= ent.y & 0xFFu;
+      const unsigned long long b = __ballot(keep != 0u);
+      if (keep) PE[out + __popcll(b & ((1ull << lane) - 1ull))] = make_uint2(ent[u].x, (ent[u].y & ~0xFFu) | keep);
+      out += __popcll(b);
     }
-    const unsigned long long b = __ballot(keep != 0u);
-    if (keep) {
-      ent.y = (ent.y & ~0xFFu) | keep;
-      PE[out + __popcll(b & ((1ull << lane) - 1ull))] = ent;
-    }
-    out += __popcll(b);
   }
   if (lane == 0) pcount[ci] = out;
 }
@@ -546,7 +567,7 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     pentries.resize((size_t)max_clusters * entry_capacity);
     pcount.resize(max_clusters);
     const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
-    hipLaunchKernelGGL(k_prune, dim3(max_clusters), dim3(64), 0, stream, n_clusters_d.p,
+    hipLaunchKernelGGL(k_prune, dim3((max_clusters + 3) / 4), dim3(256), 0, stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
                        (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur, spos.p,
                        entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
