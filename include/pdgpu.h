@@ -11,7 +11,7 @@
  *     FF_GeneralizedBorn_Still (gbff.h:123-203) and FF_Bonded (ffbonded.h), behind
  *     ForcefieldBase::calcForces / calcEnergies (forcefield.h:110-239);
  *   - the MolecularDynamics integrators (md.cpp:805-855 velocity Verlet,
- *     md.cpp:923-1059 Langevin) run as a device-resident loop.
+ *     md.cpp:858-919 Beeman, md.cpp:923-1059 Langevin) run as a device-resident loop.
  *
  * Conventions (all entry points):
  *   - PD units at the boundary: positions in Angstrom, forces in J/m, energies in
@@ -94,10 +94,11 @@ typedef struct pdgpu_gb_params {
 
 /* Integrators of MolecularDynamics (md.h:76). */
 #define PDGPU_VELOCITY_VERLET 1
+#define PDGPU_BEEMAN 2 /* PD's default integrator (md.cpp:60) */
 #define PDGPU_LANGEVIN 3
 
 typedef struct pdgpu_md_params {
-  int32_t integrator;     /* PDGPU_VELOCITY_VERLET or PDGPU_LANGEVIN */
+  int32_t integrator;     /* PDGPU_VELOCITY_VERLET, PDGPU_BEEMAN or PDGPU_LANGEVIN */
   uint32_t components;    /* PDGPU_* forcefield components evaluated each step */
   double timestep;        /* s (MolecularDynamics::Timestep) */
   double temperature;     /* K (target temperature, Langevin only) */
@@ -231,7 +232,9 @@ int pdgpu_get_born_radii(pdgpu_engine *e, double *radii);
  * device: list refresh every update_nlist steps, force evaluation, integrator.
  * If epot_trace / ekin_trace are non-NULL they receive one value per step (J):
  * the potential energy of the step's force evaluation and the kinetic energy the
- * integrator computes.  Returns PDGPU_ERR_UNSTABLE if an energy became NaN. */
+ * integrator computes.  Returns PDGPU_ERR_UNSTABLE if an energy became NaN.
+ * Beeman's old forces (PD's oldatom[i].f) live in the engine across calls and
+ * start at zero. */
 int pdgpu_md(pdgpu_engine *e, const pdgpu_md_params *p, int32_t n_steps, double *epot_trace,
              double *ekin_trace);
 

@@ -120,6 +120,8 @@ def lib():
         L.pdo_kinetic_energy.restype = C.c_double
         L.pdo_vv_step.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp, _dp]
         L.pdo_vv_step.restype = C.c_double
+        L.pdo_beeman_step.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.pdo_beeman_step.restype = C.c_double
         L.pdo_langevin_step.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_int, _dp, _dp, _dp, _dp]
         L.pdo_langevin_step.restype = C.c_double
@@ -307,6 +309,15 @@ def vv_step(mass, dt, f, pos, vel) -> float:
     assert pos.flags.c_contiguous and vel.flags.c_contiguous
     return lib().pdo_vv_step(len(mass), _f64(mass), float(dt), _f64(f).reshape(-1),
                              pos.reshape(-1), vel.reshape(-1))
+
+
+def beeman_step(mass, dt, step, steps, f, fold, pos, vel) -> float:
+    """In-place PD Beeman step (md.cpp:858-919) of a protocol of `steps` steps;
+    `fold` (the old force) is updated as PD updates oldatom[i].f.  Returns the
+    kinetic energy."""
+    assert pos.flags.c_contiguous and vel.flags.c_contiguous and fold.flags.c_contiguous
+    return lib().pdo_beeman_step(len(mass), _f64(mass), float(dt), int(step), int(steps),
+                                 _f64(f).reshape(-1), fold.reshape(-1), pos.reshape(-1), vel.reshape(-1))
 
 
 def langevin_step(mass, dt, gamma, temp, step, steps, f, noise, pos, vel) -> float:

@@ -1016,6 +1016,38 @@ double pdo_vv_step(int n, const double *mass, double dt, const double *f, double
   return ekin;
 }
 
+double pdo_beeman_step(int n, const double *mass, double dt, int step, int steps, const double *f,
+                       double *fold, double *pos, double *vel) {
+  /* applyForces_BeemanIntegration, md.cpp:858-919 (Thermostat/Barostat off).
+   * fold is the workspace's old force (oldatom[i].f), updated here. */
+  const double t = dt;
+  if (step > 0) {
+    for (int i = 0; i < n; i++) {
+      double tinvmass = t / (6.0 * mass[i]);
+      for (int d = 0; d < 3; d++) vel[3 * i + d] += ((2.0) * f[3 * i + d]) * tinvmass;
+    }
+  }
+  double ekin = pdo_kinetic_energy(n, mass, vel);
+  if (step < steps - 1) {
+    const double ia = 1 / Angstrom;
+    for (int i = 0; i < n; i++) {
+      double tinvmass = t / (6.0 * mass[i]);
+      double dr[3], dv[3];
+      for (int d = 0; d < 3; d++) {
+        dr[d] = t * (vel[3 * i + d] + (4.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass);
+        dr[d] *= ia;
+        dv[d] = (5.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass;
+      }
+      for (int d = 0; d < 3; d++) {
+        pos[3 * i + d] += dr[d];
+        vel[3 * i + d] += dv[d];
+        fold[3 * i + d] = f[3 * i + d];
+      }
+    }
+  }
+  return ekin;
+}
+
 void pdo_langevin_coeffs(double gammat, double out[6]) {
   double gt = gammat, gt2 = gt * gt, gt3 = gt * gt2, gt4 = gt2 * gt2, gt5 = gt2 * gt3,
          gt6 = gt3 * gt3, gt7 = gt3 * gt4, gt8 = gt4 * gt4, gt9 = gt4 * gt5;

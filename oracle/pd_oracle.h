@@ -165,6 +165,14 @@ double pdo_kinetic_energy(int n, const double *mass, const double *vel);
 double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
                    double *vel);
 
+/* applyForces_BeemanIntegration (md.cpp:858-919) for step `step` of `steps`,
+ * Thermostat/Barostat off: the velocity is finished with 2F(t)/6m (not on step
+ * 0), the kinetic energy taken, then (not on the last step) the position and
+ * velocity advance with (4F - F_old) and (5F - F_old) times t/6m and F becomes
+ * fold.  Returns the kinetic energy. */
+double pdo_beeman_step(int n, const double *mass, double dt, int step, int steps, const double *f,
+                       double *fold, double *pos, double *vel);
+
 /* applyForces_LangevinIntegration (md.cpp:923-1059) for step `step` of `steps`,
  * with the six unit normal deviates per atom supplied in `noise` (n1x n2y n2x n1z
  * n1y n2z, the order of the three nrand calls).  Returns the kinetic energy. */

@@ -473,7 +473,9 @@ int pdgpu_md(pdgpu_engine *h, const pdgpu_md_params *p, int32_t n_steps, double 
              double *ekin_trace) {
   return pdgpu::guarded(h, [&](Engine &E) {
     E.require(p != nullptr && n_steps >= 0, "bad md arguments");
-    E.require(p->integrator == PDGPU_VELOCITY_VERLET || p->integrator == PDGPU_LANGEVIN, "unknown integrator");
+    E.require(p->integrator == PDGPU_VELOCITY_VERLET || p->integrator == PDGPU_BEEMAN ||
+                  p->integrator == PDGPU_LANGEVIN,
+              "unknown integrator");
     E.require(p->components != 0, "no component selected");
     if (n_steps == 0) return;
     E.md(*p, n_steps, epot_trace, ekin_trace);

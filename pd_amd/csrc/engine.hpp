@@ -270,6 +270,7 @@ struct Engine {
 
   // ---- device state, original order -----------------------------------------
   DBuf<double> x, y, z, vx, vy, vz, mass;
+  DBuf<double> fox, foy, foz;  // Beeman's old forces (J/m), PD's oldatom[i].f
   DBuf<long long> fbx, fby, fbz;           // bonded / GB / integrator force accumulators
   DBuf<int> mol_first, mol_last;
   DBuf<int8_t> band;

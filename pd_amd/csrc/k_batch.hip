@@ -680,7 +680,9 @@ int pdgpu_batch_md(pdgpu_batch *h, const pdgpu_md_params *p, int32_t n_steps, do
   return bguard(h, [&](Batch &b) {
     if (!p || n_steps < 0) throw EngineError(PDGPU_ERR_ARG, "bad md arguments");
     if (p->integrator != PDGPU_VELOCITY_VERLET && p->integrator != PDGPU_LANGEVIN)
-      throw EngineError(PDGPU_ERR_ARG, "unknown integrator");
+      throw EngineError(PDGPU_ERR_ARG, p->integrator == PDGPU_BEEMAN
+                                           ? "batches run velocity Verlet or Langevin; Beeman is single-system only"
+                                           : "unknown integrator");
     if (p->timestep <= 0) throw EngineError(PDGPU_ERR_ARG, "timestep must be positive");
     b.md(*p, n_steps, epot_trace, ekin_trace);
   });

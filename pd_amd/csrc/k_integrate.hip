@@ -109,6 +109,55 @@ __device__ void block_sum4(double v0, double *out) {
   }
 }
 
+// MolecularDynamics::applyForces_BeemanIntegration (md.cpp:858-919), Thermostat
+// and Barostat off.  The velocity is finished with 2 F(t)/6m at the start of a
+// step (not on the protocol's first step) and the kinetic energy is taken; then
+// (not on its last step) the position advances by t (v + (4 F(t) - F_old) t/6m)
+// and the velocity by (5 F(t) - F_old) t/6m, and F(t) becomes the old force.
+__global__ __launch_bounds__(256) void k_beeman(
+    int n, double t, double conv, int first_kick, int move, const double *__restrict__ mass,
+    const long long *__restrict__ fbx, const long long *__restrict__ fby, const long long *__restrict__ fbz,
+    const long long *__restrict__ nbx, const long long *__restrict__ nby, const long long *__restrict__ nbz,
+    const int *__restrict__ orig_slot, double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+    double *__restrict__ vx, double *__restrict__ vy, double *__restrict__ vz, double *__restrict__ fox,
+    double *__restrict__ foy, double *__restrict__ foz, double *__restrict__ ke_partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < n) {
+    double f[3];
+    total_force(i, fbx, fby, fbz, nbx, nby, nbz, orig_slot, conv, f[0], f[1], f[2]);
+    const double m = mass[i];
+    const double tinvmass = t / (6.0 * m);  // t div 6m
+    double v[3] = {vx[i], vy[i], vz[i]};
+    if (first_kick)
+      for (int d = 0; d < 3; d++) v[d] += ((2.0) * f[d]) * tinvmass;
+    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);  // calcKineticEnergy, md.cpp:530
+    ke = 0.5 * m * (mag * mag);
+    if (move) {
+      const double fo[3] = {fox[i], foy[i], foz[i]};
+      const double ia = 1 / Consts::Angstrom;
+      double dr[3], dv[3];
+      for (int d = 0; d < 3; d++) {
+        dr[d] = t * (v[d] + (4.0 * f[d] - fo[d]) * tinvmass);
+        dr[d] *= ia;
+        dv[d] = (5.0 * f[d] - fo[d]) * tinvmass;
+      }
+      x[i] += dr[0];
+      y[i] += dr[1];
+      z[i] += dr[2];
+      for (int d = 0; d < 3; d++) v[d] += dv[d];
+      fox[i] = f[0];
+      foy[i] = f[1];
+      foz[i] = f[2];
+    }
+    vx[i] = v[0];
+    vy[i] = v[1];
+    vz[i] = v[2];
+  }
+  block_sum4(ke, ke_partial);
+}
+
 __global__ __launch_bounds__(256) void k_velocity_verlet(
     int n, double t, double conv, const double *__restrict__ mass, const long long *__restrict__ fbx,
     const long long *__restrict__ fby, const long long *__restrict__ fbz, const long long *__restrict__ nbx,
@@ -361,7 +410,19 @@ void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
   const long long *ax = nbon ? nbx.p : nullptr, *ay = nbon ? nby.p : nullptr, *az = nbon ? nbz.p : nullptr;
   const double conv = Consts::kcalA_to_Jm() / kFixScale;
   const double gammat = p.fric_coeff * p.timestep;
-  if (p.integrator == PDGPU_VELOCITY_VERLET || gammat < 0.00001) {
+  if (p.integrator == PDGPU_BEEMAN) {
+    if (fox.n < (size_t)n) {  // the workspace's old forces start at zero
+      fox.resize(n);
+      foy.resize(n);
+      foz.resize(n);
+      PDGPU_HIP(hipMemsetAsync(fox.p, 0, n * sizeof(double), stream));
+      PDGPU_HIP(hipMemsetAsync(foy.p, 0, n * sizeof(double), stream));
+      PDGPU_HIP(hipMemsetAsync(foz.p, 0, n * sizeof(double), stream));
+    }
+    hipLaunchKernelGGL(k_beeman, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, step > 0 ? 1 : 0,
+                       step < (p.total_steps - 1) ? 1 : 0, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p,
+                       x.p, y.p, z.p, vx.p, vy.p, vz.p, fox.p, foy.p, foz.p, e_partial_ke.p);
+  } else if (p.integrator == PDGPU_VELOCITY_VERLET || gammat < 0.00001) {
     // Langevin with gamma*dt < 1e-5 falls back to velocity Verlet (md.cpp:966-971)
     hipLaunchKernelGGL(k_velocity_verlet, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, mass.p, fbx.p,
                        fby.p, fbz.p, ax, ay, az, orig_slot.p, x.p, y.p, z.p, vx.p, vy.p, vz.p, e_partial_ke.p);

@@ -21,6 +21,7 @@ PDGPU_GB = 0x4
 PDGPU_SASA = 0x8
 PDGPU_ENERGY_ONLY = 0x100
 PDGPU_VELOCITY_VERLET = 1
+PDGPU_BEEMAN = 2
 PDGPU_LANGEVIN = 3
 
 ERRORS = {-1: "argument", -2: "hip", -3: "setup", -4: "nomem", -5: "unstable"}

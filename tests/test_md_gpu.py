@@ -52,6 +52,31 @@ def test_velocity_verlet_step_bit_exact():
     assert ek[0] == pytest.approx(ke, rel=1e-14)
 
 
+def test_beeman_steps_bit_exact():
+    """Three Beeman steps of a 3-step protocol (no kick on step 0, no move on the
+    last) against the oracle fed the engine's own forces; the old forces start at
+    zero and persist between calls."""
+    s = S.water_box(512, seed=6)
+    e = G.engine_for(s, G.nb_params(**WATER_NB), padding=2.0, bonded=True)
+    e.build_list()
+    v0 = mb_velocities(s, 300.0, 3)
+    e.set_velocities(v0)
+    comps = G.PDGPU_NONBONDED | G.PDGPU_BONDED
+    pos = e.get_positions()
+    vel = v0.copy()
+    fold = np.zeros_like(pos)
+    for st in range(3):
+        print(f"case beeman step {st}", flush=True)
+        f, _ = e.compute(comps)
+        md = G.MDParams(integrator=G.PDGPU_BEEMAN, components=comps, timestep=1e-15,
+                        update_nlist=10, step_offset=st, total_steps=3, seed=1)
+        ep, ek = e.md(md, 1)
+        ke = O.beeman_step(s.mass, 1e-15, st, 3, f, fold, pos, vel)
+        assert np.array_equal(e.get_positions(), pos)
+        assert np.array_equal(e.get_velocities(), vel)
+        assert ek[0] == pytest.approx(ke, rel=1e-14)
+
+
 def test_langevin_step_matches_oracle():
     s = S.peptide_chain(10, seed=2)
     e = G.engine_for(s, None, bonded=True)

@@ -626,3 +626,68 @@ def test_lcpo_energy_routine_clamps_negative_overlaps():
     _, sasa_f, _ = O.lcpo_forces(pos, np.ones(2, np.int8), rad2, P2, np.full(2, 1e-21), c2, pk2)
     assert sasa_e[0] == pytest.approx(0.6 * 4 * math.pi * R1 ** 2, rel=1e-14)
     assert sasa_f[0] == pytest.approx(0.6 * 4 * math.pi * R1 ** 2 - 0.2 * s12, rel=1e-14)
+
+
+def _beeman_oscillator(steps, fold0):
+    k, m, dt = 500.0, np.array([2.0e-26]), 1e-15  # N/m, kg, s: a 40 fs period
+    pos = np.array([[0.3, -0.1, 0.2]])
+    vel = np.array([[150.0, -80.0, 40.0]])
+    fold = np.array(fold0, dtype=np.float64).reshape(1, 3)
+    xs, vs, fs, kes, folds = [pos.copy()], [vel.copy()], [], [], [fold.copy()]
+    for st in range(steps):
+        f = -k * pos * 1e-10  # J/m, positions in A
+        fs.append(f.copy())
+        kes.append(O.beeman_step(m, dt, st, steps, f, fold, pos, vel))
+        xs.append(pos.copy()); vs.append(vel.copy()); folds.append(fold.copy())
+    return m, dt, np.array(xs)[:, 0], np.array(vs)[:, 0], np.array(fs)[:, 0], np.array(kes), np.array(folds)[:, 0]
+
+
+def test_beeman_positions_follow_the_verlet_recurrence():
+    """Beeman's positions obey x(n+1) - 2 x(n) + x(n-1) = a(n) dt^2 from step 1 on,
+    whatever the starting old force: the velocity terms (2a + 5a_old - a_older)/6
+    and the position terms (4a - a_old)/6 cancel to exactly a.  A wrong coefficient
+    in either velocity update or in the position update breaks it."""
+    m, dt, xs, vs, fs, kes, folds = _beeman_oscillator(40, [1e-12, -2e-12, 5e-13])
+    for n in range(1, 39):
+        lhs = xs[n + 1] - 2 * xs[n] + xs[n - 1]
+        rhs = fs[n] / m[0] * dt * dt / 1e-10
+        np.testing.assert_allclose(lhs, rhs, rtol=1e-9, atol=1e-12 * np.abs(rhs).max())
+
+
+def test_beeman_first_and_last_step():
+    """Step 0 takes no velocity kick; the last step kicks and then neither moves
+    nor touches the old force (md.cpp:877, 894)."""
+    m, dt, xs, vs, fs, kes, folds = _beeman_oscillator(3, [1e-12, -2e-12, 5e-13])
+    v0 = np.array([150.0, -80.0, 40.0])
+    assert kes[0] == pytest.approx(0.5 * m[0] * (v0 @ v0), rel=1e-15)
+    # step 0 moves with the supplied old force: dr = t (v + (4F - F_old) t/6m)
+    dr0 = dt * (v0 + (4 * fs[0] - np.array([1e-12, -2e-12, 5e-13])) * dt / (6 * m[0])) / 1e-10
+    np.testing.assert_allclose(xs[1] - xs[0], dr0, rtol=1e-12)
+    np.testing.assert_array_equal(folds[1], fs[0])
+    # last step (index 2): velocity gets 2F/6m t, position and old force unchanged
+    np.testing.assert_array_equal(xs[3], xs[2])
+    np.testing.assert_array_equal(folds[3], folds[2])
+    np.testing.assert_allclose(vs[3] - vs[2], 2 * fs[2] * dt / (6 * m[0]), rtol=1e-12)
+
+
+def test_beeman_tracks_the_harmonic_oscillator():
+    """With a consistent start (old force = F at x(-dt) of the exact solution), the
+    Beeman velocities at kick time follow the exact oscillator to O(dt^2) over the
+    first third of a period (later the O(dt^2) frequency error of the Verlet
+    positions dominates)."""
+    k, m, dt = 500.0, 2.0e-26, 1e-15
+    w = np.sqrt(k / m)
+    A = 0.3e-10  # m
+    x0 = A; v0 = 0.0
+    pos = np.array([[x0 / 1e-10, 0.0, 0.0]]); vel = np.array([[v0, 0.0, 0.0]])
+    fold = np.array([[-k * A * np.cos(-w * dt), 0.0, 0.0]])
+    steps = 14
+    err = []
+    for st in range(steps):
+        f = -k * pos * 1e-10
+        ke = O.beeman_step(np.array([m]), dt, st, steps, f, fold, pos, vel)
+        t_kick = st * dt
+        v_exact = -A * w * np.sin(w * t_kick)
+        ke_exact = 0.5 * m * v_exact ** 2
+        err.append(abs(ke - ke_exact) / (0.5 * k * A * A))
+    assert max(err) < 5e-3  # (w dt)^2 = 0.025; Beeman's velocity error is a fraction of that
