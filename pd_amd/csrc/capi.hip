@@ -22,6 +22,8 @@ Engine::Engine(int natoms, int dev) : n(natoms), device(dev) {
   sp_off.upload(off, stream);
   sp_part.resize(1);
   sp_ord.resize(1);
+  PDGPU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_list_check), 4 * sizeof(unsigned long long)));
+  PDGPU_HIP(hipEventCreateWithFlags(&list_event, hipEventDisableTiming));
 }
 
 Engine::~Engine() {
@@ -32,6 +34,8 @@ Engine::~Engine() {
       (void)hipEventDestroy(pr.second);
     }
   for (hipEvent_t ev : event_pool) (void)hipEventDestroy(ev);
+  if (list_event) (void)hipEventDestroy(list_event);
+  if (h_list_check) (void)hipHostFree(h_list_check);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -450,11 +454,14 @@ int pdgpu_compute(pdgpu_engine *h, uint32_t components, double *f_accum, size_t 
 
 int pdgpu_set_timing(pdgpu_engine *h, int32_t enable) {
   return pdgpu::guarded(h, [&](Engine &E) {
+    E.require(enable >= 0, "timing interval must be >= 0");
     E.resolve_timers();
     E.timing = enable != 0;
+    E.timing_every = enable > 0 ? enable : 1;
     for (auto &kv : E.timers) {
       kv.second.ms = 0.0;
       kv.second.launches = 0;
+      kv.second.calls = 0;
     }
   });
 }

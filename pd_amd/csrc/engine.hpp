@@ -311,6 +311,12 @@ struct Engine {
   DBuf<unsigned long long> entry_counter;  // [0] entries used, [1] overflow flag
   size_t entry_capacity = 0;
   int n_clusters = 0;                      // host copy after a build
+  // The rebuild inside pdgpu_md reads its counters back into pinned memory
+  // without waiting; the step checks them once its force kernels are queued.
+  unsigned long long *h_list_check = nullptr;  // [0] entries, [1] overflow, [2] clusters
+  hipEvent_t list_event = nullptr;
+  bool list_check_pending = false;
+  bool finish_list_check();                // false: the list overflowed (capacity already grown)
 
   // ---- reductions ---------------------------------------------------------------
   DBuf<double> e_partial;                  // per-wave nonbonded energy partials (4 per wave)
@@ -344,15 +350,21 @@ struct Engine {
   void sync() { PDGPU_HIP(hipStreamSynchronize(stream)); }
 
   // list.hip
-  void build_list();
+  void build_list(bool deferred = false);  // deferred: the overflow check waits for finish_list_check
   void export_pairs(std::vector<int> &counts, std::vector<uint32_t> &packed);
   // kernel timing instrumentation (HIP events on `stream`)
   bool timing = false;
   struct TimerSlot {
     double ms = 0.0;
-    long long launches = 0;
+    long long launches = 0;  // launches timed
+    long long calls = 0;     // launches seen while timing was on
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   };
+  int timing_every = 1;  // time every k-th launch of each named kernel
+  bool timer_sample(const char *name) {
+    if (!timing) return false;
+    return (timers[name].calls++ % timing_every) == 0;
+  }
   std::map<std::string, TimerSlot> timers;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
@@ -389,15 +401,16 @@ struct Engine {
 struct ScopedTimer {
   Engine &E;
   const char *name;
+  bool on;
   hipEvent_t a = nullptr, b = nullptr;
-  ScopedTimer(Engine &e, const char *n) : E(e), name(n) {
-    if (E.timing) {
+  ScopedTimer(Engine &e, const char *n) : E(e), name(n), on(e.timer_sample(n)) {
+    if (on) {
       a = E.take_event();
       (void)hipEventRecord(a, E.stream);
     }
   }
   ~ScopedTimer() {
-    if (E.timing) {
+    if (on) {
       b = E.take_event();
       (void)hipEventRecord(b, E.stream);
       auto &slot = E.timers[name];
@@ -411,15 +424,16 @@ struct ScopedTimer {
 struct ScopedTimerOpt {
   Engine *E;
   const char *name;
+  bool on;
   hipEvent_t a = nullptr;
-  ScopedTimerOpt(Engine *e, const char *n) : E(e), name(n) {
-    if (E && E->timing) {
+  ScopedTimerOpt(Engine *e, const char *n) : E(e), name(n), on(e && e->timer_sample(n)) {
+    if (on) {
       a = E->take_event();
       (void)hipEventRecord(a, E->stream);
     }
   }
   ~ScopedTimerOpt() {
-    if (E && E->timing) {
+    if (on) {
       hipEvent_t b = E->take_event();
       (void)hipEventRecord(b, E->stream);
       E->timers[name].pending.emplace_back(a, b);

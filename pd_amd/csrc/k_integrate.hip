@@ -494,8 +494,13 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
   const double cor = (comp & PDGPU_NONBONDED) ? nbk.vdwcor_per_atom * n : 0.0;
   for (int s = 0; s < n_steps; s++) {
     const int step = p.step_offset + s;
-    if ((comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0) build_list();
+    const bool rebuild = (comp & (PDGPU_NONBONDED)) && p.update_nlist > 0 && (step % p.update_nlist) == 0;
+    if (rebuild) build_list(true);
     evaluate(comp, step, false);
+    if (rebuild && !finish_list_check()) {  // the list overflowed: rebuild with room and evaluate again
+      build_list();
+      evaluate(comp, step, false);
+    }
     integrate(p, step, e_out.p + 16);
     const StepRecord rec{trace.p, s, (comp & PDGPU_NONBONDED) ? 1 : 0, (comp & PDGPU_BONDED) ? 1 : 0,
                          (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor, Consts::kcal_to_J()};

@@ -631,7 +631,7 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-void Engine::build_list() {
+void Engine::build_list(bool deferred) {
   require(have_positions, "positions not set");
   require(list_cutoff > 0.0, "no cutoff requested from the neighbour list");
   const int T = 256;
@@ -772,18 +772,39 @@ void Engine::build_list() {
                          entries.p, cl_entry_count.p, wg_ncand.p, wg_cand.p, entry_counter.p);
     }
     PDGPU_HIP(hipGetLastError());
-    unsigned long long hc[2];
-    entry_counter.download(hc, 2, stream);
-    n_clusters_d.download(&n_clusters, 1, stream);
+    PDGPU_HIP(hipMemcpyAsync(h_list_check, entry_counter.p, 2 * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, stream));
+    PDGPU_HIP(hipMemcpyAsync(reinterpret_cast<int *>(h_list_check + 2), n_clusters_d.p, sizeof(int),
+                             hipMemcpyDeviceToHost, stream));
+    if (deferred) {
+      PDGPU_HIP(hipEventRecord(list_event, stream));
+      list_check_pending = true;
+      break;
+    }
     sync();
-    if (hc[1] == 0) break;
+    n_clusters = *reinterpret_cast<const int *>(h_list_check + 2);
+    const unsigned long long over = h_list_check[1];
+    if (over == 0) break;
     if (attempt == 3) throw EngineError(PDGPU_ERR_SETUP, "neighbour list does not fit (too many neighbours)");
-    entry_capacity = std::max(entry_capacity * 2, (size_t)(hc[1] * 1.25 + 64) / 64 * 64);
+    entry_capacity = std::max(entry_capacity * 2, (size_t)(over * 1.25 + 64) / 64 * 64);
   }
   full_update_count++;
   prune_forced = true;
   prepare_slots();
   list_valid = true;
+}
+
+bool Engine::finish_list_check() {
+  if (!list_check_pending) return true;
+  PDGPU_HIP(hipEventSynchronize(list_event));
+  list_check_pending = false;
+  n_clusters = *reinterpret_cast<const int *>(h_list_check + 2);
+  const unsigned long long over = h_list_check[1];
+  if (over == 0) return true;
+  entry_capacity = std::max(entry_capacity * 2, (size_t)(over * 1.25 + 64) / 64 * 64);
+  list_valid = false;
+  full_update_count--;  // the synchronous rebuild that follows counts this update
+  return false;
 }
 
 // PD's packed per-atom form of the pair set (neighbourlist.h:131-151), built on

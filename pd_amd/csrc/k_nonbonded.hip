@@ -307,9 +307,11 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
         const float rinv = __builtin_amdgcn_rsqf(r2f);
         const float r = r2f * rinv;
         const float rinv2 = rinv * rinv;
+        // A pair that is off has eps = qq = 0, so every energy and force term below
+        // comes out as (signed) zero without a select of its own.
         float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
         if (VDW) {
-          const float eps = (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z;
+          const float eps = (on && inv) ? (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z : 0.0f;
           const float Rij = pa.y + qj.y;
           const float s2 = Rij * Rij * rinv2;
           const float B = s2 * s2 * s2;
@@ -321,12 +323,11 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           const float dS = -P.invw2_vdw4 * t * S0;
           const float S = S0 * S0;
           d0 = S * d0 + e0 * dS;
-          e0 = e0 * S;
-          Ev = inv ? e0 : 0.0f;
-          dEdr += inv ? d0 : 0.0f;
+          Ev = e0 * S;
+          dEdr += d0;
         }
         if (ELEC != EL_NONE) {
-          const float qq = (s14 ? P.elec14 : 1.0f) * pa.x * qj.x;
+          const float qq = on ? (s14 ? P.elec14 : 1.0f) * pa.x * qj.x : 0.0f;
           if (ELEC == EL_PLAIN) {
             Ee = qq * rinv;
             dEdr += -Ee * rinv;
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           }
         }
         if (FORCES) {
-          const float sc = on ? dEdr * rinv : 0.0f;
+          const float sc = dEdr * rinv;
           const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
           fix[a] += gx;
           fiy[a] += gy;
@@ -361,9 +362,9 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           fjy -= gy;
           fjz -= gz;
         }
-        ev8 += on ? Ev : 0.0f;
-        ee8 += on ? Ee : 0.0f;
-        vir8 += on ? r * dEdr : 0.0f;
+        ev8 += Ev;
+        ee8 += Ee;
+        vir8 += r * dEdr;
       }
       ev += (double)ev8;
       ee += (double)ee8;

@@ -348,8 +348,9 @@ class Engine:
         self._chk(self.L.pdgpu_get_born_radii(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
-    def set_timing(self, on: bool):
-        self._chk(self.L.pdgpu_set_timing(self.h, int(bool(on))))
+    def set_timing(self, on: bool, every: int = 1):
+        """Time every `every`-th launch of each named kernel while on."""
+        self._chk(self.L.pdgpu_set_timing(self.h, int(every) if on else 0))
 
     def kernel_time(self, name: str):
         """(total ms, launches) of a named kernel since timing was enabled."""

@@ -77,6 +77,37 @@ def test_beeman_steps_bit_exact():
         assert ek[0] == pytest.approx(ke, rel=1e-14)
 
 
+def test_md_recovers_from_list_overflow_at_a_rebuild():
+    """A rebuild inside pdgpu_md whose list outgrows the capacity sized at the first
+    build (here from a 4x expanded chain) is redone with room before the step is
+    integrated: the run equals one that started from the compact chain."""
+    s = S.peptide_chain(20, seed=11)
+    kw = dict(Cutoff=12.0, InnerCutoff=10.0, VdwCutoff=12.0, VdwInnerCutoff=10.0, ForceSwitch=0,
+              EnergySwitch=1)
+    comps = G.PDGPU_NONBONDED | G.PDGPU_BONDED
+    sparse = s.copy()
+    c = s.pos.mean(axis=0)
+    sparse.pos = c + 4.0 * (s.pos - c)
+    a = G.engine_for(sparse, G.nb_params(**kw), padding=0.5, bonded=True)
+    a.build_list()
+    a.set_positions(s.pos)
+    b = G.engine_for(s, G.nb_params(**kw), padding=0.5, bonded=True)
+    v0 = mb_velocities(s, 300.0, 5)
+    a.set_velocities(v0)
+    b.set_velocities(v0)
+    md = G.MDParams(integrator=G.PDGPU_VELOCITY_VERLET, components=comps, timestep=1e-15, update_nlist=2,
+                    step_offset=0, total_steps=5, seed=1)
+    print("case overflow retry", flush=True)
+    a.set_timing(True)
+    b.set_timing(True)
+    ea, ka = a.md(md, 5)
+    eb, kb = b.md(md, 5)
+    assert a.kernel_time("pair_search")[1] == b.kernel_time("pair_search")[1] + 1  # the retried build
+    assert a.full_update_count() == b.full_update_count() + 1  # a's build before the run
+    assert np.array_equal(a.get_positions(), b.get_positions())
+    assert np.array_equal(ea, eb) and np.array_equal(ka, kb)
+
+
 def test_langevin_step_matches_oracle():
     s = S.peptide_chain(10, seed=2)
     e = G.engine_for(s, None, bonded=True)
