@@ -99,13 +99,13 @@ __global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict
     }
     const float r2f = on ? (float)r2 : 1.0f;
     const float Rs = Ri + sR[jj];
-    const float rdivR = r2f / (Rs * Rs);
+    const float rdivR = r2f * __builtin_amdgcn_rcpf(Rs * Rs);  // hardware reciprocal, 1 ulp
     float ccf = 1.0f;
     if (rdivR <= K.invP5) {
       const float c = 0.5f - 0.5f * cosf(rdivR * K.P5 * K.pi);
       ccf = c * c;
     }
-    const float base = K.P4 * ccf / (r2f * r2f);
+    const float base = K.P4 * ccf * __builtin_amdgcn_rcpf(r2f * r2f);
     gi += on ? base * sV[jj] : 0.0f;
     acc[jj] += on ? base * Vi : 0.0f;
   }
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const float aj = sa[jj];
     const float qq = qi * sq[jj];
     const float aiaj = ai * aj;
-    const float uGB = -r2f * 0.25f / aiaj;
+    const float uGB = -r2f * 0.25f * __builtin_amdgcn_rcpf(aiaj);
     const bool nb = bo >= 3;                      // vdW/elec only for non-bonded and 1-4 pairs
     const float s14e = bo == 3 ? K.elec14 : 1.0f;
     // switching (r >= InnerCutoff), gbff.cpp:1640-1651
@@ -213,12 +213,12 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const float Sw = S0 * S0;
     float ep = 0.0f, dedr = 0.0f, dal = 0.0f, Eel = 0.0f, Fel = 0.0f;
     if (EONLY) {
-      ep = on_gb ? K.premul * qq / sqrtf(r2f + aiaj * __expf(-r2f * 0.25f / aiaj)) * Sw : 0.0f;
+      ep = on_gb ? K.premul * qq * __builtin_amdgcn_rsqf(r2f + aiaj * __expf(uGB)) * Sw : 0.0f;
       if (nb) Eel = K.ec_in * s14e * qq * rinv * Sw;
     } else if (uGB > K.thresh) {
       const float tGB = __expf(uGB);
-      const float lGB = 1.0f / (r2f + tGB * aiaj);
-      const float fGB = sqrtf(lGB);
+      const float fGB = __builtin_amdgcn_rsqf(r2f + tGB * aiaj);  // 1 / f_GB, 1 ulp
+      const float lGB = fGB * fGB;
       ep = K.premul * qq * fGB;
       dedr = -ep * (r - 0.25f * r * tGB) * lGB;
       dal = -ep * tGB * (1.0f - uGB) * lGB;
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
     }
     const float r2f = on ? (float)r2 : 1.0f;
     const float Rs = Ri + sR[jj];
-    const float rdivR = r2f / (Rs * Rs);
+    const float rdivR = r2f * __builtin_amdgcn_rcpf(Rs * Rs);
     float ccf = 1.0f, dccf = 0.0f;
     if (rdivR <= K.invP5) {
       const float th = rdivR * K.P5 * K.pi;
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
     }
     // both directions of the pair: gpi deda_i P4 Vj + gpj deda_j P4 Vi
     const float r6 = r2f * r2f * r2f;
-    const float dedr = on ? (Wi * sV[jj] + sW[jj] * Vi) * (4.0f * ccf - dccf) / r6 : 0.0f;
+    const float dedr = on ? (Wi * sV[jj] + sW[jj] * Vi) * (4.0f * ccf - dccf) * __builtin_amdgcn_rcpf(r6) : 0.0f;
     const float gx = dedr * (float)dx, gy = dedr * (float)dy, gz = dedr * (float)dz;
     fix -= gx;  // dedi_i += dedr (p_j - p_i); force = -dedi
     fiy -= gy;
