@@ -100,6 +100,8 @@ def lib():
         L.pdo_build_list.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, C.c_int,
                                      C.c_int, _bp, C.c_int, _ip, _ip, _ip, _ip, _up, C.c_int64]
         L.pdo_build_list.restype = C.c_int64
+        L.pdo_build_list_cells.argtypes = L.pdo_build_list.argtypes
+        L.pdo_build_list_cells.restype = C.c_int64
         L.pdo_nb_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _up, _dp,
                                     C.POINTER(NBParams), _dp, C.POINTER(Energies)]
         L.pdo_nb_abs_energy.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _ip, _up, _dp, C.POINTER(NBParams)]
@@ -202,8 +204,10 @@ def bond_order(sys, max_bond_order: int = 7) -> BondOrderData:
     return BondOrderData(mid, band, si[:ns].copy(), sj[:ns].copy(), so[:ns].copy())
 
 
-def build_list(sys, pos, cutoff: float, padding: float, calc_shadow: bool, bo: BondOrderData):
-    """NeighbourList::calcNewList (neighbourlist.cpp:222-244).  Returns (counts, packed)."""
+def build_list(sys, pos, cutoff: float, padding: float, calc_shadow: bool, bo: BondOrderData,
+               cells: bool = False):
+    """NeighbourList::calcNewList (neighbourlist.cpp:222-244).  Returns (counts, packed).
+    cells=True takes the candidates from a cell grid (same result, O(N))."""
     n = sys.n
     periodic = sys.box is not None
     box = _f64(sys.box if periodic else [0, 0, 0])
@@ -217,7 +221,8 @@ def build_list(sys, pos, cutoff: float, padding: float, calc_shadow: bool, bo: B
     spo = bo.sp_order if sp_n else np.zeros(1, np.int32)
     while True:
         packed = np.zeros(cap, np.uint32)
-        tot = L.pdo_build_list(n, p, int(periodic), box, float(cutoff), float(padding),
+        fn = L.pdo_build_list_cells if cells else L.pdo_build_list
+        tot = fn(n, p, int(periodic), box, float(cutoff), float(padding),
                                int(calc_shadow), bo.mid, bo.band, sp_n, spi, spj, spo, counts,
                                packed, cap)
         if tot >= 0:

@@ -195,6 +195,39 @@ static void correct_off_diagonal(int n_sparse, const int *sp_i, const int *sp_j,
   }
 }
 
+/* One candidate pair of calcNewList (neighbourlist.cpp:380-422): PD's FP64
+ * displacement, `while` minimum image, r^2 test and bond-order tag.  Returns 1
+ * and the packed entry when j is listed with i. */
+static int list_pair(int i, int j, const double *pos, int periodic, const double box[3],
+                     const double half[3], double lim, int mid, const signed char *band, uint32_t *out) {
+  double dx = pos[3 * j + 0] - pos[3 * i + 0];
+  double dy = pos[3 * j + 1] - pos[3 * i + 1];
+  double dz = pos[3 * j + 2] - pos[3 * i + 2];
+  int image = 13;
+  if (periodic) {
+    while (dx > half[0]) { dx -= box[0]; image -= 9; }
+    while (dx < -half[0]) { dx += box[0]; image += 9; }
+    while (dy > half[1]) { dy -= box[1]; image -= 3; }
+    while (dy < -half[1]) { dy += box[1]; image += 3; }
+    while (dz > half[2]) { dz -= box[2]; image -= 1; }
+    while (dz < -half[2]) { dz += box[2]; image += 1; }
+  }
+  double r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 > lim) return 0;
+  int bo = periodic ? 15 : 7;
+  if (abs(j - i) < mid) {
+    if (j > i)
+      bo = band[(long)i * mid + (j - i - 1)];
+    else
+      bo = band[(long)j * mid + (i - j - 1)];
+    if (j == i) return 0;
+  }
+  uint32_t e = ((uint32_t)j & 0x00FFFFFFu) | ((uint32_t)(bo & 7) << 24);
+  if (periodic) e |= (uint32_t)(image & 31) << 27;
+  *out = e;
+  return 1;
+}
+
 int64_t pdo_build_list(int n, const double *pos, int periodic, const double box[3], double cutoff,
                        double padding, int calc_shadow, int mid, const signed char *band,
                        int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
@@ -213,36 +246,132 @@ int64_t pdo_build_list(int n, const double *pos, int periodic, const double box[
     counts[i] = 0;
     int limit = calc_shadow ? n : i;
     for (int j = 0; j < limit; j++) {
-      double dx = pos[3 * j + 0] - pos[3 * i + 0];
-      double dy = pos[3 * j + 1] - pos[3 * i + 1];
-      double dz = pos[3 * j + 2] - pos[3 * i + 2];
-      int image = 13;
-      if (periodic) {
-        while (dx > half[0]) { dx -= box[0]; image -= 9; }
-        while (dx < -half[0]) { dx += box[0]; image += 9; }
-        while (dy > half[1]) { dy -= box[1]; image -= 3; }
-        while (dy < -half[1]) { dy += box[1]; image += 3; }
-        while (dz > half[2]) { dz -= box[2]; image -= 1; }
-        while (dz < -half[2]) { dz += box[2]; image += 1; }
-      }
-      double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > lim) continue;
-      int bo = periodic ? 15 : 7;
-      if (abs(j - i) < mid) {
-        if (j > i)
-          bo = band[(long)i * mid + (j - i - 1)];
-        else
-          bo = band[(long)j * mid + (i - j - 1)];
-        if (j == i) continue;
-      }
-      uint32_t e = ((uint32_t)j & 0x00FFFFFFu) | ((uint32_t)(bo & 7) << 24);
-      if (periodic) e |= (uint32_t)(image & 31) << 27;
+      uint32_t e;
+      if (!list_pair(i, j, pos, periodic, box, half, lim, mid, band, &e)) continue;
       if (total < cap) packed[total] = e;
       total++;
       counts[i]++;
     }
   }
   start[n] = total;
+  if (total > cap) {
+    free(start);
+    return -total;
+  }
+  correct_off_diagonal(n_sparse, sp_i, sp_j, sp_order, calc_shadow, start, counts, packed);
+  free(start);
+  return total;
+}
+
+static int cmp_entry_j(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a & 0x00FFFFFFu, y = *(const uint32_t *)b & 0x00FFFFFFu;
+  return (x > y) - (x < y);
+}
+
+int64_t pdo_build_list_cells(int n, const double *pos, int periodic, const double box[3], double cutoff,
+                             double padding, int calc_shadow, int mid, const signed char *band,
+                             int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
+                             int *counts, uint32_t *packed, int64_t cap) {
+  /* The same list as pdo_build_list from a cell grid: cells at least
+   * Cutoff+Padding wide, so every pair within it under the minimum image lies in
+   * neighbouring cells; each candidate takes list_pair's exact test, and the
+   * entries kept are put in ascending j, the order PD's loop visits them. */
+  const double rl = cutoff + padding;
+  const double lim = sqr(rl);
+  double half[3] = {0, 0, 0}, lo[3], ext[3];
+  int nc[3];
+  if (periodic) {
+    for (int d = 0; d < 3; d++) {
+      half[d] = box[d] / 2.0;
+      lo[d] = 0.0;
+      ext[d] = box[d];
+    }
+  } else {
+    for (int d = 0; d < 3; d++) {
+      double mn = 1e300, mx = -1e300;
+      for (int i = 0; i < n; i++) {
+        if (pos[3 * i + d] < mn) mn = pos[3 * i + d];
+        if (pos[3 * i + d] > mx) mx = pos[3 * i + d];
+      }
+      lo[d] = mn;
+      ext[d] = (mx - mn) + 1e-9;
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    nc[d] = (int)(ext[d] / rl);
+    if (nc[d] < 1) nc[d] = 1;
+    if (nc[d] > 1024) nc[d] = 1024;
+  }
+  const int ncell = nc[0] * nc[1] * nc[2];
+  int *cell = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *head = (int *)malloc(sizeof(int) * (size_t)(ncell + 1));
+  int *order = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int c = 0; c <= ncell; c++) head[c] = 0;
+  for (int i = 0; i < n; i++) {
+    int ci[3];
+    for (int d = 0; d < 3; d++) {
+      double v = pos[3 * i + d] - lo[d];
+      if (periodic) v -= box[d] * floor(v / box[d]);
+      int k = (int)(v / ext[d] * nc[d]);
+      if (k < 0) k = 0;
+      if (k >= nc[d]) k = nc[d] - 1;
+      ci[d] = k;
+    }
+    cell[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
+    head[cell[i] + 1]++;
+  }
+  for (int c = 0; c < ncell; c++) head[c + 1] += head[c];
+  {
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(ncell > 0 ? ncell : 1));
+    for (int c = 0; c < ncell; c++) fill[c] = head[c];
+    for (int i = 0; i < n; i++) order[fill[cell[i]]++] = i;  /* ascending i inside a cell */
+    free(fill);
+  }
+  uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  int64_t total = 0;
+  int64_t *start = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  for (int i = 0; i < n; i++) {
+    start[i] = total;
+    counts[i] = 0;
+    const int c = cell[i];
+    const int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+    int nn = 0;
+    int seen[27], nseen = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int x = cx + dx, y = cy + dy, z = cz + dz;
+          if (periodic) {
+            x = (x + nc[0]) % nc[0];
+            y = (y + nc[1]) % nc[1];
+            z = (z + nc[2]) % nc[2];
+          } else if (x < 0 || y < 0 || z < 0 || x >= nc[0] || y >= nc[1] || z >= nc[2]) {
+            continue;
+          }
+          const int cc = (z * nc[1] + y) * nc[0] + x;
+          int dup = 0;  /* fewer than 3 cells along an axis: neighbours repeat */
+          for (int k = 0; k < nseen; k++) dup |= seen[k] == cc;
+          if (dup) continue;
+          seen[nseen++] = cc;
+          for (int k = head[cc]; k < head[cc + 1]; k++) {
+            const int j = order[k];
+            uint32_t e;
+            if ((calc_shadow ? j != i : j < i) && list_pair(i, j, pos, periodic, box, half, lim, mid, band, &e))
+              cand[nn++] = e;
+          }
+        }
+    qsort(cand, (size_t)nn, sizeof(uint32_t), cmp_entry_j);
+    for (int k = 0; k < nn; k++) {
+      if (total < cap) packed[total] = cand[k];
+      total++;
+      counts[i]++;
+    }
+  }
+  start[n] = total;
+  free(cand);
+  free(order);
+  free(head);
+  free(cell);
   if (total > cap) {
     free(start);
     return -total;

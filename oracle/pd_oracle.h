@@ -105,6 +105,14 @@ int64_t pdo_build_list(int n, const double *pos, int periodic, const double box[
                        int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
                        int *counts, uint32_t *packed, int64_t cap);
 
+/* The same list (entries, order and tags) from a cell grid of cells at least
+ * Cutoff+Padding wide: O(N) candidates, each given pdo_build_list's exact test. */
+int64_t pdo_build_list_cells(int n, const double *pos, int periodic, const double box[3],
+                       double cutoff, double padding, int calc_shadow,
+                       int mid, const signed char *band,
+                       int n_sparse, const int *sp_i, const int *sp_j, const int *sp_order,
+                       int *counts, uint32_t *packed, int64_t cap);
+
 /* FF_NonBonded::calcForces -> FF_NonBonded_CalcForces_T_fast2 (nonbonded.cpp:559-910,
  * dispatch 1222-1246).  Adds forces into f (AoS, J/m).  Energies follow PD: epot_vdw
  * and epot_elec are assigned, epot and virial are added to. */

@@ -691,3 +691,26 @@ def test_beeman_tracks_the_harmonic_oscillator():
         ke_exact = 0.5 * m * v_exact ** 2
         err.append(abs(ke - ke_exact) / (0.5 * k * A * A))
     assert max(err) < 5e-3  # (w dt)^2 = 0.025; Beeman's velocity error is a fraction of that
+
+
+@pytest.mark.parametrize("case", ["water216", "water512_shadow", "peptide_vacuum", "small_box"])
+def test_cell_list_equals_all_pairs_list(case):
+    """The cell-grid candidates give PD's all-pairs list exactly: entries, order,
+    images and bond-order tags (periodic half and shadow lists, vacuum, and a box
+    of fewer than three cells per axis where neighbour cells repeat)."""
+    if case == "peptide_vacuum":
+        s = S.peptide_chain(12, seed=4)
+        cut, pad, shadow = 12.0, 0.5, False
+    elif case == "small_box":
+        s = S.water_box(64, seed=8)  # 12.4 A box, 2 cells per axis at 5 A
+        cut, pad, shadow = 4.5, 0.5, False
+    else:
+        s = S.water_box(216 if case == "water216" else 512, seed=3)
+        cut, pad, shadow = 8.0, 1.0, case.endswith("shadow")
+    pos = O.clean_space(s)
+    bo = O.bond_order(s)
+    c1, p1 = O.build_list(s, pos, cut, pad, shadow, bo)
+    c2, p2 = O.build_list(s, pos, cut, pad, shadow, bo, cells=True)
+    assert p1.size > 0
+    np.testing.assert_array_equal(c1, c2)
+    np.testing.assert_array_equal(p1, p2)
