@@ -11,7 +11,9 @@
  *     FF_GeneralizedBorn_Still (gbff.h:123-203) and FF_Bonded (ffbonded.h), behind
  *     ForcefieldBase::calcForces / calcEnergies (forcefield.h:110-239);
  *   - the MolecularDynamics integrators (md.cpp:805-855 velocity Verlet,
- *     md.cpp:858-919 Beeman, md.cpp:923-1059 Langevin) run as a device-resident loop.
+ *     md.cpp:858-919 Beeman, md.cpp:923-1059 Langevin), its Berendsen and
+ *     Andersen thermostats (md.cpp:579-660) and momentum removal
+ *     (md.cpp:263-272) run as a device-resident loop.
  *
  * Conventions (all entry points):
  *   - PD units at the boundary: positions in Angstrom, forces in J/m, energies in
@@ -97,17 +99,27 @@ typedef struct pdgpu_gb_params {
 #define PDGPU_BEEMAN 2 /* PD's default integrator (md.cpp:60) */
 #define PDGPU_LANGEVIN 3
 
+/* Thermostats of MolecularDynamics (md.h:96); they act in velocity Verlet and
+ * Beeman steps (Langevin regulates the temperature itself). */
+#define PDGPU_NO_THERMOSTAT 0
+#define PDGPU_ANDERSEN 1
+#define PDGPU_BERENDSEN 2
+
 typedef struct pdgpu_md_params {
   int32_t integrator;     /* PDGPU_VELOCITY_VERLET, PDGPU_BEEMAN or PDGPU_LANGEVIN */
   uint32_t components;    /* PDGPU_* forcefield components evaluated each step */
   double timestep;        /* s (MolecularDynamics::Timestep) */
-  double temperature;     /* K (target temperature, Langevin only) */
+  double temperature;     /* K (target temperature: Langevin and the thermostats) */
   double fric_coeff;      /* 1/s (MolecularDynamics::FricCoeff) */
   int32_t update_nlist;   /* MolecularDynamics::UpdateNList (list rebuild period) */
   int32_t step_offset;    /* first step number (Langevin's first/last step guards) */
   int32_t total_steps;    /* MolecularDynamics::Steps */
-  uint64_t seed;          /* Langevin noise: Philox4x32-10 key */
+  uint64_t seed;          /* Langevin / Andersen noise: Philox4x32-10 key */
   uint32_t noise_stream;  /* Philox counter word 2; a batch adds the system index */
+  int32_t thermostat;     /* MolecularDynamics::Thermostat: PDGPU_NO_THERMOSTAT (default), ... */
+  int32_t remove_momentum; /* UpdateRemoveTotalMomentum (PD: 200); only with a thermostat, md.cpp:263-272 */
+  double berendsen_tau;   /* s (BerendsenTau, PD: 100e-15) */
+  double andersen_rate;   /* AndersenRate (PD: 0.1) */
 } pdgpu_md_params;
 
 /* ---- lifecycle -------------------------------------------------------------- */

@@ -122,12 +122,24 @@ def lib():
         L.pdo_kinetic_energy.restype = C.c_double
         L.pdo_vv_step.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp, _dp]
         L.pdo_vv_step.restype = C.c_double
+        L.pdo_vv_kick.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp]
+        L.pdo_vv_move.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp, _dp]
+        L.pdo_beeman_kick.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp]
+        L.pdo_beeman_move.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.pdo_beeman_step.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.pdo_beeman_step.restype = C.c_double
         L.pdo_langevin_step.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_int, _dp, _dp, _dp, _dp]
         L.pdo_langevin_step.restype = C.c_double
         L.pdo_langevin_coeffs.argtypes = [C.c_double, _dp]
+        L.pdo_berendsen.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.pdo_berendsen.restype = C.c_double
+        L.pdo_andersen_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.POINTER(C.c_uint32), _dp]
+        L.pdo_andersen.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                   C.c_uint32, _dp]
+        L.pdo_andersen.restype = C.c_int
+        L.pdo_zero_momentum.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.pdo_remd_accept.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
         L.pdo_remd_accept.restype = C.c_int
         L.pdo_philox4x32.argtypes = [_up, _up, _up]
@@ -316,6 +328,23 @@ def vv_step(mass, dt, f, pos, vel) -> float:
                              pos.reshape(-1), vel.reshape(-1))
 
 
+def vv_kick(mass, dt, f, vel):
+    lib().pdo_vv_kick(len(mass), _f64(mass), float(dt), _f64(f).reshape(-1), vel.reshape(-1))
+
+
+def vv_move(mass, dt, f, pos, vel):
+    lib().pdo_vv_move(len(mass), _f64(mass), float(dt), _f64(f).reshape(-1), pos.reshape(-1), vel.reshape(-1))
+
+
+def beeman_kick(mass, dt, step, f, vel):
+    lib().pdo_beeman_kick(len(mass), _f64(mass), float(dt), int(step), _f64(f).reshape(-1), vel.reshape(-1))
+
+
+def beeman_move(mass, dt, step, steps, f, fold, pos, vel):
+    lib().pdo_beeman_move(len(mass), _f64(mass), float(dt), int(step), int(steps), _f64(f).reshape(-1),
+                          fold.reshape(-1), pos.reshape(-1), vel.reshape(-1))
+
+
 def beeman_step(mass, dt, step, steps, f, fold, pos, vel) -> float:
     """In-place PD Beeman step (md.cpp:858-919) of a protocol of `steps` steps;
     `fold` (the old force) is updated as PD updates oldatom[i].f.  Returns the
@@ -323,6 +352,35 @@ def beeman_step(mass, dt, step, steps, f, fold, pos, vel) -> float:
     assert pos.flags.c_contiguous and vel.flags.c_contiguous and fold.flags.c_contiguous
     return lib().pdo_beeman_step(len(mass), _f64(mass), float(dt), int(step), int(steps),
                                  _f64(f).reshape(-1), fold.reshape(-1), pos.reshape(-1), vel.reshape(-1))
+
+
+def berendsen(ekin, temp, dt, tau, vel) -> float:
+    """In-place applyBerendsenThermostat (md.cpp:601-617); returns chi."""
+    assert vel.flags.c_contiguous
+    return lib().pdo_berendsen(len(vel), float(ekin), float(temp), float(dt), float(tau), vel.reshape(-1))
+
+
+def andersen_draws(seed, stream, step, atom):
+    t = C.c_uint32()
+    z = np.zeros(4)
+    lib().pdo_andersen_draws(int(seed), int(stream), int(step), int(atom), C.byref(t), z)
+    return t.value, z
+
+
+def andersen(mass, temp, rate, dt, seed, stream, step, vel) -> int:
+    """In-place applyAndersonThermostat (md.cpp:620-660) on the Philox draws."""
+    assert vel.flags.c_contiguous
+    return lib().pdo_andersen(len(mass), _f64(mass), float(temp), float(rate), float(dt), int(seed),
+                              int(stream), int(step), vel.reshape(-1))
+
+
+def zero_momentum(mass, pos, vel):
+    """In-place momentum removal of run_core with a thermostat (md.cpp:263-272, 412-517);
+    returns (linear, angular) momentum before the removal."""
+    assert vel.flags.c_contiguous
+    out = np.zeros(6)
+    lib().pdo_zero_momentum(len(mass), _f64(mass), _f64(pos).reshape(-1), vel.reshape(-1), out)
+    return out[:3], out[3:]
 
 
 def langevin_step(mass, dt, gamma, temp, step, steps, f, noise, pos, vel) -> float:

@@ -1119,9 +1119,8 @@ double pdo_kinetic_energy(int n, const double *mass, const double *vel) {
   return ekin;
 }
 
-double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
-                   double *vel) {
-  /* applyForces_VelocityVerletIntegration, md.cpp:805-855 */
+void pdo_vv_kick(int n, const double *mass, double dt, const double *f, double *vel) {
+  /* applyForces_VelocityVerletIntegration, first loop (md.cpp:814-826) */
   const double t = dt;
   for (int i = 0; i < n; i++) {
     double tinvmass = 0.5 * t / mass[i];
@@ -1129,7 +1128,11 @@ double pdo_vv_step(int n, const double *mass, double dt, const double *f, double
     vel[3 * i + 1] += f[3 * i + 1] * tinvmass;
     vel[3 * i + 2] += f[3 * i + 2] * tinvmass;
   }
-  double ekin = pdo_kinetic_energy(n, mass, vel);
+}
+
+void pdo_vv_move(int n, const double *mass, double dt, const double *f, double *pos, double *vel) {
+  /* second loop (md.cpp:836-854) */
+  const double t = dt;
   const double ia = 1 / Angstrom;
   for (int i = 0; i < n; i++) {
     double tinvmass = 0.5 * t / mass[i];
@@ -1142,38 +1145,55 @@ double pdo_vv_step(int n, const double *mass, double dt, const double *f, double
       vel[3 * i + d] += dv;
     }
   }
+}
+
+double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
+                   double *vel) {
+  /* applyForces_VelocityVerletIntegration, md.cpp:805-855 */
+  pdo_vv_kick(n, mass, dt, f, vel);
+  double ekin = pdo_kinetic_energy(n, mass, vel);
+  pdo_vv_move(n, mass, dt, f, pos, vel);
   return ekin;
+}
+
+void pdo_beeman_kick(int n, const double *mass, double dt, int step, const double *f, double *vel) {
+  /* applyForces_BeemanIntegration, md.cpp:877-889: not on the first step */
+  const double t = dt;
+  if (step <= 0) return;
+  for (int i = 0; i < n; i++) {
+    double tinvmass = t / (6.0 * mass[i]);
+    for (int d = 0; d < 3; d++) vel[3 * i + d] += ((2.0) * f[3 * i + d]) * tinvmass;
+  }
+}
+
+void pdo_beeman_move(int n, const double *mass, double dt, int step, int steps, const double *f,
+                     double *fold, double *pos, double *vel) {
+  /* md.cpp:894-918: not on the last step; F becomes the old force */
+  const double t = dt;
+  if (step >= steps - 1) return;
+  const double ia = 1 / Angstrom;
+  for (int i = 0; i < n; i++) {
+    double tinvmass = t / (6.0 * mass[i]);
+    double dr[3], dv[3];
+    for (int d = 0; d < 3; d++) {
+      dr[d] = t * (vel[3 * i + d] + (4.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass);
+      dr[d] *= ia;
+      dv[d] = (5.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass;
+    }
+    for (int d = 0; d < 3; d++) {
+      pos[3 * i + d] += dr[d];
+      vel[3 * i + d] += dv[d];
+      fold[3 * i + d] = f[3 * i + d];
+    }
+  }
 }
 
 double pdo_beeman_step(int n, const double *mass, double dt, int step, int steps, const double *f,
                        double *fold, double *pos, double *vel) {
-  /* applyForces_BeemanIntegration, md.cpp:858-919 (Thermostat/Barostat off).
-   * fold is the workspace's old force (oldatom[i].f), updated here. */
-  const double t = dt;
-  if (step > 0) {
-    for (int i = 0; i < n; i++) {
-      double tinvmass = t / (6.0 * mass[i]);
-      for (int d = 0; d < 3; d++) vel[3 * i + d] += ((2.0) * f[3 * i + d]) * tinvmass;
-    }
-  }
+  /* applyForces_BeemanIntegration, md.cpp:858-919 (Thermostat/Barostat off) */
+  pdo_beeman_kick(n, mass, dt, step, f, vel);
   double ekin = pdo_kinetic_energy(n, mass, vel);
-  if (step < steps - 1) {
-    const double ia = 1 / Angstrom;
-    for (int i = 0; i < n; i++) {
-      double tinvmass = t / (6.0 * mass[i]);
-      double dr[3], dv[3];
-      for (int d = 0; d < 3; d++) {
-        dr[d] = t * (vel[3 * i + d] + (4.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass);
-        dr[d] *= ia;
-        dv[d] = (5.0 * f[3 * i + d] - fold[3 * i + d]) * tinvmass;
-      }
-      for (int d = 0; d < 3; d++) {
-        pos[3 * i + d] += dr[d];
-        vel[3 * i + d] += dv[d];
-        fold[3 * i + d] = f[3 * i + d];
-      }
-    }
-  }
+  pdo_beeman_move(n, mass, dt, step, steps, f, fold, pos, vel);
   return ekin;
 }
 
@@ -1314,6 +1334,113 @@ void pdo_langevin_noise(uint64_t seed, uint32_t stream, uint32_t step, uint32_t 
     uint32_t w[4];
     pdo_philox4x32(ctr, key, w);
     pdo_box_muller(u53(w[0], w[1]), u53(w[2], w[3]), &out[2 * k], &out[2 * k + 1]);
+  }
+}
+
+/* ---- thermostats and momentum removal ------------------------------------------ */
+
+double pdo_berendsen(int n, double ekin, double temp, double dt, double tau, double *vel) {
+  /* applyBerendsenThermostat, md.cpp:601-617 */
+  double Edes = 3.0 * n * kB * temp / 2.0;
+  double chi = sqrt(1.0 + dt * (Edes / ekin - 1.0) / tau);
+  for (int i = 0; i < 3 * n; i++) vel[i] *= chi;
+  return chi;
+}
+
+void pdo_andersen_draws(uint64_t seed, uint32_t stream, uint32_t step, uint32_t atom, uint32_t *test,
+                        double z[4]) {
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t a[4], b[4], c[4];
+  uint32_t ca[4] = {atom, step, stream, 0x414E0000u}, cb[4] = {atom, step, stream, 0x414E0001u},
+           cc[4] = {atom, step, stream, 0x414E0002u};
+  pdo_philox4x32(ca, key, a);
+  pdo_philox4x32(cb, key, b);
+  pdo_philox4x32(cc, key, c);
+  *test = a[0] % 10000u;
+  z[0] = u53(a[2], a[3]);
+  z[1] = u53(b[0], b[1]);
+  z[2] = u53(b[2], b[3]);
+  z[3] = u53(c[0], c[1]);
+}
+
+int pdo_andersen(int n, const double *mass, double temp, double rate_param, double dt, uint64_t seed,
+                 uint32_t stream, uint32_t step, double *vel) {
+  /* applyAndersonThermostat, md.cpp:620-660; rand() % 10000 and frand() come
+   * from pdo_andersen_draws */
+  const double TwoPI = 2.0 * PD_PI;
+  double rate = rate_param * dt * 1E15 / pow((double)n, (double)(2.0 / 3.0));
+  int intrate = (int)(10000.0 * rate);
+  int changed = 0;
+  for (int i = 0; i < n; i++) {
+    uint32_t test;
+    double z[4];
+    pdo_andersen_draws(seed, stream, step, (uint32_t)i, &test, z);
+    if ((int)test > intrate) continue;
+    double sigma = sqrt(kB * temp / mass[i]);
+    double l = sqrt(-2.0 * log(z[0]));
+    double vx = l * cos((-TwoPI) * z[1]) * sigma;
+    double vy = l * sin((-TwoPI) * z[1]) * sigma;
+    l = sqrt(-2.0 * log(z[2]));
+    vy = l * cos((-TwoPI) * z[3]) * sigma;  /* nrand(vy, vz): vy is drawn again */
+    double vz = l * sin((-TwoPI) * z[3]) * sigma;
+    if (fabs(vx) > (sigma * 4)) continue;
+    if (fabs(vy) > (sigma * 4)) continue;
+    if (fabs(vz) > (sigma * 4)) continue;
+    vel[3 * i + 0] = vx;
+    vel[3 * i + 1] = vy;
+    vel[3 * i + 2] = vz;
+    changed++;
+  }
+  return changed;
+}
+
+void pdo_zero_momentum(int n, const double *mass, const double *pos, double *vel, double out[6]) {
+  /* calcTotalLinearMomentum + calcTotalAngularMomentum, then
+   * zeroTotalLinearMomentum + zeroTotalAngularMomentum (md.cpp:412-517) */
+  double linmom[3] = {0, 0, 0}, totmass = 0;
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) linmom[d] += vel[3 * i + d] * mass[i];
+    totmass += mass[i];
+  }
+  double com[3] = {0, 0, 0}, M = 0;
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) com[d] += pos[3 * i + d] * mass[i];
+    M += mass[i];
+  }
+  for (int d = 0; d < 3; d++) com[d] /= M;
+  double L[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++) {
+    double p[3];
+    for (int d = 0; d < 3; d++) p[d] = (pos[3 * i + d] - com[d]) * Angstrom;
+    const double *v = vel + 3 * i;
+    L[0] += (p[1] * v[2] - p[2] * v[1]) * mass[i];
+    L[1] += (p[2] * v[0] - p[0] * v[2]) * mass[i];
+    L[2] += (p[0] * v[1] - p[1] * v[0]) * mass[i];
+  }
+  (void)totmass;
+  const double nm1 = (double)(n - 1);  /* (iend - istart) */
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) vel[3 * i + d] -= linmom[d] / (nm1 * mass[i]);
+  const double Lmag = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  for (int i = 0; i < n; i++) {
+    double p[3], t[3];
+    for (int d = 0; d < 3; d++) p[d] = (pos[3 * i + d] - com[d]) * Angstrom;
+    t[0] = L[1] * p[2] - L[2] * p[1];
+    t[1] = L[2] * p[0] - L[0] * p[2];
+    t[2] = L[0] * p[1] - L[1] * p[0];
+    double tm = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    if (tm != 0)
+      for (int d = 0; d < 3; d++) t[d] /= tm;
+    double pm = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    for (int d = 0; d < 3; d++) {
+      t[d] *= Lmag / pm;
+      t[d] /= nm1 * mass[i];
+      vel[3 * i + d] -= t[d];
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    out[d] = linmom[d];
+    out[3 + d] = L[d];
   }
 }
 

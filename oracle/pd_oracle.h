@@ -168,6 +168,15 @@ void pdo_torsion_forces(const double *pos, int nt, const int *ti, const int *ta,
 /* MolecularDynamics::calcKineticEnergy (md.cpp:521-556). */
 double pdo_kinetic_energy(int n, const double *mass, const double *vel);
 
+/* The two halves of the velocity Verlet and Beeman steps, between which PD
+ * takes the kinetic energy and applies the thermostat (md.cpp:831-833, 891-893):
+ * the velocity kick of the first loop, and the second loop that moves. */
+void pdo_vv_kick(int n, const double *mass, double dt, const double *f, double *vel);
+void pdo_vv_move(int n, const double *mass, double dt, const double *f, double *pos, double *vel);
+void pdo_beeman_kick(int n, const double *mass, double dt, int step, const double *f, double *vel);
+void pdo_beeman_move(int n, const double *mass, double dt, int step, int steps, const double *f,
+                     double *fold, double *pos, double *vel);
+
 /* applyForces_VelocityVerletIntegration (md.cpp:805-855), Thermostat/Barostat
  * off.  Returns the kinetic energy computed between the two loops. */
 double pdo_vv_step(int n, const double *mass, double dt, const double *f, double *pos,
@@ -187,6 +196,29 @@ double pdo_beeman_step(int n, const double *mass, double dt, int step, int steps
 double pdo_langevin_step(int n, const double *mass, double dt, double gamma, double temp,
                          int step, int steps, const double *f, const double *noise,
                          double *pos, double *vel);
+
+/* applyBerendsenThermostat (md.cpp:601-617): every velocity times
+ * chi = sqrt(1 + dt (Edes/ekin - 1) / tau), Edes = 3 N kB T / 2.  Returns chi. */
+double pdo_berendsen(int n, double ekin, double temp, double dt, double tau, double *vel);
+
+/* The draws that stand in for PD's rand() in the Andersen thermostat: for atom
+ * `atom` of step `step`, the test value (rand() % 10000) and four uniforms in
+ * (0,1] for the two nrand calls, from Philox4x32-10 counters
+ * (atom, step, stream, 0x414E0000 + k). */
+void pdo_andersen_draws(uint64_t seed, uint32_t stream, uint32_t step, uint32_t atom, uint32_t *test,
+                        double z[4]);
+
+/* applyAndersonThermostat (md.cpp:620-660) on those draws.  Returns the number
+ * of atoms given a new velocity. */
+int pdo_andersen(int n, const double *mass, double temp, double rate_param, double dt, uint64_t seed,
+                 uint32_t stream, uint32_t step, double *vel);
+
+/* The momentum step of run_core with a thermostat (md.cpp:263-272):
+ * calcTotalLinearMomentum and calcTotalAngularMomentum, then
+ * zeroTotalLinearMomentum and zeroTotalAngularMomentum (md.cpp:412-517), both
+ * dividing by (iend - istart) = N - 1.  out = linear momentum (kg m/s) and
+ * angular momentum (kg m^2/s) before the removal. */
+void pdo_zero_momentum(int n, const double *mass, const double *pos, double *vel, double out[6]);
 
 /* Langevin coefficients c0 c1 c2 varr varv crv (md.cpp:947-986). */
 void pdo_langevin_coeffs(double gammat, double out[6]);

@@ -483,6 +483,10 @@ int pdgpu_md(pdgpu_engine *h, const pdgpu_md_params *p, int32_t n_steps, double 
     E.require(p->integrator == PDGPU_VELOCITY_VERLET || p->integrator == PDGPU_BEEMAN ||
                   p->integrator == PDGPU_LANGEVIN,
               "unknown integrator");
+    E.require(p->thermostat == PDGPU_NO_THERMOSTAT || p->thermostat == PDGPU_ANDERSEN ||
+                  p->thermostat == PDGPU_BERENDSEN,
+              "unknown thermostat");
+    E.require(p->thermostat != PDGPU_BERENDSEN || p->berendsen_tau > 0, "BerendsenTau must be positive");
     E.require(p->components != 0, "no component selected");
     if (n_steps == 0) return;
     E.md(*p, n_steps, epot_trace, ekin_trace);

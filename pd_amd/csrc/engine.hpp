@@ -271,6 +271,9 @@ struct Engine {
   // ---- device state, original order -----------------------------------------
   DBuf<double> x, y, z, vx, vy, vz, mass;
   DBuf<double> fox, foy, foz;  // Beeman's old forces (J/m), PD's oldatom[i].f
+  DBuf<double> thermo;         // [0] Berendsen's chi of the step
+  DBuf<double> mom_part, mom;  // momentum-step partials and totals
+  void remove_momentum();      // run_core's momentum step with a thermostat
   DBuf<long long> fbx, fby, fbz;           // bonded / GB / integrator force accumulators
   DBuf<int> mol_first, mol_last;
   DBuf<int8_t> band;

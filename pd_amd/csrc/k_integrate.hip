@@ -70,7 +70,7 @@ __global__ __launch_bounds__(1024) void k_sums(SumList L) {
     if (R.use_bonded) ep += E[4] + E[5] + E[6];
     if (R.use_gb) ep += E[8] + E[9] + E[10] + E[12];
     R.trace[2 * R.step] = ep * R.kcal_to_J;
-    R.trace[2 * R.step + 1] = E[16];
+    R.trace[2 * R.step + 1] = E[20];
   }
 }
 
@@ -237,6 +237,248 @@ __device__ void langevin_noise(uint32_t k0, uint32_t k1, uint32_t stream, uint32
   }
 }
 
+// ---- thermostats (md.cpp:579-660) and momentum removal (md.cpp:263-272, 412-517) ----
+// With a thermostat, velocity Verlet and Beeman run as two kernels around one
+// reduction: the velocity kick and the kinetic energy (calcKineticEnergy),
+// then applyThermostat on the velocities and the moving half of the step.
+struct ThermoArgs {
+  int kind;          // PDGPU_NO_THERMOSTAT / PDGPU_ANDERSEN / PDGPU_BERENDSEN
+  double temp, dt, tau;
+  int intrate;       // Andersen: (int)(10000 * AndersenRate * dt 1e15 / N^(2/3))
+  uint32_t k0, k1, stream, step;
+};
+
+template <int INTEG>  // PDGPU_VELOCITY_VERLET or PDGPU_BEEMAN
+__global__ __launch_bounds__(256) void k_kick(int n, double t, double conv, int first_kick,
+                                              const double *__restrict__ mass, const long long *__restrict__ fbx,
+                                              const long long *__restrict__ fby, const long long *__restrict__ fbz,
+                                              const long long *__restrict__ nbx, const long long *__restrict__ nby,
+                                              const long long *__restrict__ nbz, const int *__restrict__ orig_slot,
+                                              double *__restrict__ vx, double *__restrict__ vy,
+                                              double *__restrict__ vz, double *__restrict__ ke_partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (i < n) {
+    double f[3];
+    total_force(i, fbx, fby, fbz, nbx, nby, nbz, orig_slot, conv, f[0], f[1], f[2]);
+    const double m = mass[i];
+    double v[3] = {vx[i], vy[i], vz[i]};
+    if (INTEG == PDGPU_VELOCITY_VERLET) {
+      const double tinvmass = 0.5 * t / m;
+      for (int d = 0; d < 3; d++) v[d] += f[d] * tinvmass;
+    } else if (first_kick) {
+      const double tinvmass = t / (6.0 * m);
+      for (int d = 0; d < 3; d++) v[d] += ((2.0) * f[d]) * tinvmass;
+    }
+    const double mag = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    ke = 0.5 * m * (mag * mag);
+    vx[i] = v[0];
+    vy[i] = v[1];
+    vz[i] = v[2];
+  }
+  block_sum4(ke, ke_partial);
+}
+
+// The kinetic energy of the step (one workgroup) and Berendsen's factor
+// chi = sqrt(1 + dt (Edes/ekin - 1) / tau), Edes = 3 N kB T / 2 (md.cpp:601-617).
+__global__ __launch_bounds__(1024) void k_thermo_sum(const double *__restrict__ part, int count, int natoms,
+                                                     ThermoArgs A, double *__restrict__ ke_slot,
+                                                     double *__restrict__ chi) {
+  __shared__ double s[16];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < count; i += 1024) a += part[4 * i];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ekin = 0.0;
+    for (int k = 0; k < 16; k++) ekin += s[k];
+    ke_slot[0] = ekin;
+    ke_slot[1] = ke_slot[2] = ke_slot[3] = 0.0;
+    const double Edes = 3.0 * natoms * Consts::kB * A.temp / 2.0;
+    chi[0] = A.kind == PDGPU_BERENDSEN ? sqrt(1.0 + A.dt * (Edes / ekin - 1.0) / A.tau) : 1.0;
+  }
+}
+
+// The four draws that stand in for PD's rand() of atom `atom`: the test value
+// (rand() % 10000) and uniforms in (0,1] for the two nrand calls.
+__device__ void andersen_draws(uint32_t k0, uint32_t k1, uint32_t stream, uint32_t step, uint32_t atom,
+                               uint32_t &test, double z[4]) {
+  uint32_t a[4] = {atom, step, stream, 0x414E0000u}, b[4] = {atom, step, stream, 0x414E0001u},
+           c[4] = {atom, step, stream, 0x414E0002u};
+  philox(a, k0, k1);
+  philox(b, k0, k1);
+  philox(c, k0, k1);
+  test = a[0] % 10000u;
+  z[0] = u53(a[2], a[3]);
+  z[1] = u53(b[0], b[1]);
+  z[2] = u53(b[2], b[3]);
+  z[3] = u53(c[0], c[1]);
+}
+
+template <int INTEG>
+__global__ __launch_bounds__(256) void k_thermo_move(
+    int n, double t, double conv, int move, ThermoArgs A, const double *__restrict__ chi,
+    const double *__restrict__ mass, const long long *__restrict__ fbx, const long long *__restrict__ fby,
+    const long long *__restrict__ fbz, const long long *__restrict__ nbx, const long long *__restrict__ nby,
+    const long long *__restrict__ nbz, const int *__restrict__ orig_slot, double *__restrict__ x,
+    double *__restrict__ y, double *__restrict__ z, double *__restrict__ vx, double *__restrict__ vy,
+    double *__restrict__ vz, double *__restrict__ fox, double *__restrict__ foy, double *__restrict__ foz) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+  double v[3] = {vx[i], vy[i], vz[i]};
+  if (A.kind == PDGPU_BERENDSEN) {
+    const double c = chi[0];
+    for (int d = 0; d < 3; d++) v[d] *= c;
+  } else if (A.kind == PDGPU_ANDERSEN) {  // applyAndersonThermostat, md.cpp:620-660
+    uint32_t test;
+    double zz[4];
+    andersen_draws(A.k0, A.k1, A.stream, A.step, (uint32_t)i, test, zz);
+    if (!((int)test > A.intrate)) {
+      const double TwoPI = 2.0 * Consts::PI;
+      const double sigma = sqrt(Consts::kB * A.temp / m);
+      double l = sqrt(-2.0 * log(zz[0]));
+      const double nx = l * cos((-TwoPI) * zz[1]) * sigma;
+      l = sqrt(-2.0 * log(zz[2]));
+      const double ny = l * cos((-TwoPI) * zz[3]) * sigma;  // nrand(vy, vz) draws vy again
+      const double nz = l * sin((-TwoPI) * zz[3]) * sigma;
+      if (!(fabs(nx) > (sigma * 4)) && !(fabs(ny) > (sigma * 4)) && !(fabs(nz) > (sigma * 4))) {
+        v[0] = nx;
+        v[1] = ny;
+        v[2] = nz;
+      }
+    }
+  }
+  double f[3];
+  total_force(i, fbx, fby, fbz, nbx, nby, nbz, orig_slot, conv, f[0], f[1], f[2]);
+  const double ia = 1 / Consts::Angstrom;
+  if (INTEG == PDGPU_VELOCITY_VERLET) {  // second loop, md.cpp:836-854
+    const double tinvmass = 0.5 * t / m;
+    double dr[3];
+    for (int d = 0; d < 3; d++) {
+      const double dv = f[d] * tinvmass;
+      v[d] += dv;
+      dr[d] = t * (v[d] + dv);
+      dr[d] *= ia;
+      v[d] += dv;
+    }
+    x[i] += dr[0];
+    y[i] += dr[1];
+    z[i] += dr[2];
+  } else if (move) {  // Beeman, md.cpp:894-918
+    const double tinvmass = t / (6.0 * m);
+    const double fo[3] = {fox[i], foy[i], foz[i]};
+    double dr[3], dv[3];
+    for (int d = 0; d < 3; d++) {
+      dr[d] = t * (v[d] + (4.0 * f[d] - fo[d]) * tinvmass);
+      dr[d] *= ia;
+      dv[d] = (5.0 * f[d] - fo[d]) * tinvmass;
+    }
+    x[i] += dr[0];
+    y[i] += dr[1];
+    z[i] += dr[2];
+    for (int d = 0; d < 3; d++) v[d] += dv[d];
+    fox[i] = f[0];
+    foy[i] = f[1];
+    foz[i] = f[2];
+  }
+  vx[i] = v[0];
+  vy[i] = v[1];
+  vz[i] = v[2];
+}
+
+// Sums for the momentum step: mass, m v, m p, m (p x v) (p in A), 10 per block.
+__global__ __launch_bounds__(256) void k_mom_partials(int n, const double *__restrict__ mass,
+                                                      const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ z, const double *__restrict__ vx,
+                                                      const double *__restrict__ vy, const double *__restrict__ vz,
+                                                      double *__restrict__ part) {
+  __shared__ double s[4][10];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const double m = mass[i], p[3] = {x[i], y[i], z[i]}, v[3] = {vx[i], vy[i], vz[i]};
+    a[0] = m;
+    for (int d = 0; d < 3; d++) {
+      a[1 + d] = v[d] * m;
+      a[4 + d] = p[d] * m;
+    }
+    a[7] = (p[1] * v[2] - p[2] * v[1]) * m;
+    a[8] = (p[2] * v[0] - p[0] * v[2]) * m;
+    a[9] = (p[0] * v[1] - p[1] * v[0]) * m;
+  }
+  for (int k = 0; k < 10; k++) a[k] = wave_sum(a[k]);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 10; k++) s[threadIdx.x >> 6][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x < 10) part[10 * blockIdx.x + threadIdx.x] = s[0][threadIdx.x] + s[1][threadIdx.x] +
+                                                                s[2][threadIdx.x] + s[3][threadIdx.x];
+}
+
+// One workgroup: mom = linear momentum (3), angular momentum (3), centre of mass
+// (3, A), |L|.  L = A (sum m p x v - com x P), the sum of md.cpp:466-475.
+__global__ __launch_bounds__(256) void k_mom_finish(const double *__restrict__ part, int count,
+                                                    double *__restrict__ mom) {
+  __shared__ double s[4][10];
+  double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < count; b += 256)
+    for (int k = 0; k < 10; k++) a[k] += part[10 * b + k];
+  for (int k = 0; k < 10; k++) a[k] = wave_sum(a[k]);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 10; k++) s[threadIdx.x >> 6][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[10];
+    for (int k = 0; k < 10; k++) t[k] = s[0][k] + s[1][k] + s[2][k] + s[3][k];
+    const double M = t[0], P[3] = {t[1], t[2], t[3]};
+    const double com[3] = {t[4] / M, t[5] / M, t[6] / M};
+    double L[3];
+    L[0] = (t[7] - (com[1] * P[2] - com[2] * P[1])) * Consts::Angstrom;
+    L[1] = (t[8] - (com[2] * P[0] - com[0] * P[2])) * Consts::Angstrom;
+    L[2] = (t[9] - (com[0] * P[1] - com[1] * P[0])) * Consts::Angstrom;
+    for (int d = 0; d < 3; d++) {
+      mom[d] = P[d];
+      mom[3 + d] = L[d];
+      mom[6 + d] = com[d];
+    }
+    mom[9] = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  }
+}
+
+// zeroTotalLinearMomentum then zeroTotalAngularMomentum (md.cpp:480-517), both
+// dividing by (iend - istart) = N - 1.
+__global__ __launch_bounds__(256) void k_mom_apply(int n, const double *__restrict__ mass,
+                                                   const double *__restrict__ x, const double *__restrict__ y,
+                                                   const double *__restrict__ z, double *__restrict__ vx,
+                                                   double *__restrict__ vy, double *__restrict__ vz,
+                                                   const double *__restrict__ mom) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i], nm1 = (double)(n - 1);
+  double v[3] = {vx[i], vy[i], vz[i]};
+  for (int d = 0; d < 3; d++) v[d] -= mom[d] / (nm1 * m);
+  const double L[3] = {mom[3], mom[4], mom[5]};
+  const double p[3] = {(x[i] - mom[6]) * Consts::Angstrom, (y[i] - mom[7]) * Consts::Angstrom,
+                       (z[i] - mom[8]) * Consts::Angstrom};
+  double t[3] = {L[1] * p[2] - L[2] * p[1], L[2] * p[0] - L[0] * p[2], L[0] * p[1] - L[1] * p[0]};
+  const double tm = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  if (tm != 0)
+    for (int d = 0; d < 3; d++) t[d] /= tm;
+  const double pm = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  for (int d = 0; d < 3; d++) {
+    t[d] *= mom[9] / pm;
+    t[d] /= nm1 * m;
+    v[d] -= t[d];
+  }
+  vx[i] = v[0];
+  vy[i] = v[1];
+  vz[i] = v[2];
+}
+
 struct LangevinArgs {
   double t, gamma, gammat, temp;
   double c0, c1, c2, varr, varv, crv;
@@ -324,7 +566,7 @@ void Engine::flush_sums(const StepRecord *rec) {
 
 // e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
 // bond, angle, torsion, virial; [8..11] GB pol, vdw, elec, virial (see k_gb.hip);
-// [12] GB self; [13] LCPO cavity energy; [16] kinetic energy (J); others spare.
+// [12] GB self; [13..16] LCPO cavity energy (slot 13); [20] kinetic energy (J); others spare.
 void Engine::evaluate(uint32_t components, int step, bool flush) {
   const bool eonly = (components & PDGPU_ENERGY_ONLY) != 0;
   const bool nbon = (components & PDGPU_NONBONDED) != 0;
@@ -410,20 +652,55 @@ void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
   const long long *ax = nbon ? nbx.p : nullptr, *ay = nbon ? nby.p : nullptr, *az = nbon ? nbz.p : nullptr;
   const double conv = Consts::kcalA_to_Jm() / kFixScale;
   const double gammat = p.fric_coeff * p.timestep;
-  if (p.integrator == PDGPU_BEEMAN) {
-    if (fox.n < (size_t)n) {  // the workspace's old forces start at zero
-      fox.resize(n);
-      foy.resize(n);
-      foz.resize(n);
-      PDGPU_HIP(hipMemsetAsync(fox.p, 0, n * sizeof(double), stream));
-      PDGPU_HIP(hipMemsetAsync(foy.p, 0, n * sizeof(double), stream));
-      PDGPU_HIP(hipMemsetAsync(foz.p, 0, n * sizeof(double), stream));
-    }
+  const bool beeman = p.integrator == PDGPU_BEEMAN;
+  // Langevin with gamma*dt < 1e-5 falls back to velocity Verlet (md.cpp:966-971)
+  const bool vv = !beeman && (p.integrator == PDGPU_VELOCITY_VERLET || gammat < 0.00001);
+  if (beeman && fox.n < (size_t)n) {  // the workspace's old forces start at zero
+    fox.resize(n);
+    foy.resize(n);
+    foz.resize(n);
+    PDGPU_HIP(hipMemsetAsync(fox.p, 0, n * sizeof(double), stream));
+    PDGPU_HIP(hipMemsetAsync(foy.p, 0, n * sizeof(double), stream));
+    PDGPU_HIP(hipMemsetAsync(foz.p, 0, n * sizeof(double), stream));
+  }
+  if ((beeman || vv) && p.thermostat != PDGPU_NO_THERMOSTAT) {
+    // kick + kinetic energy, applyThermostat, then the moving half (md.cpp:831-833, 891-893)
+    ThermoArgs A{};
+    A.kind = p.thermostat;
+    A.temp = p.temperature;
+    A.dt = p.timestep;
+    A.tau = p.berendsen_tau;
+    const double rate = p.andersen_rate * p.timestep * 1E15 / std::pow(double(n), double(2.0 / 3.0));
+    A.intrate = (int)(10000.0 * rate);
+    A.k0 = (uint32_t)p.seed;
+    A.k1 = (uint32_t)(p.seed >> 32);
+    A.stream = p.noise_stream;
+    A.step = (uint32_t)step;
+    thermo.resize(1);
+    const int first = step > 0 ? 1 : 0, move = step < (p.total_steps - 1) ? 1 : 0;
+    if (beeman)
+      hipLaunchKernelGGL(k_kick<PDGPU_BEEMAN>, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, first, mass.p,
+                         fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p, vx.p, vy.p, vz.p, e_partial_ke.p);
+    else
+      hipLaunchKernelGGL(k_kick<PDGPU_VELOCITY_VERLET>, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, first,
+                         mass.p, fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p, vx.p, vy.p, vz.p, e_partial_ke.p);
+    hipLaunchKernelGGL(k_thermo_sum, dim3(1), dim3(1024), 0, stream, e_partial_ke.p, grid, n, A, ke_slot, thermo.p);
+    if (beeman)
+      hipLaunchKernelGGL(k_thermo_move<PDGPU_BEEMAN>, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, move, A,
+                         thermo.p, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p, x.p, y.p, z.p, vx.p, vy.p,
+                         vz.p, fox.p, foy.p, foz.p);
+    else
+      hipLaunchKernelGGL(k_thermo_move<PDGPU_VELOCITY_VERLET>, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv,
+                         move, A, thermo.p, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p, x.p, y.p, z.p,
+                         vx.p, vy.p, vz.p, nullptr, nullptr, nullptr);
+    PDGPU_HIP(hipGetLastError());
+    return;
+  }
+  if (beeman) {
     hipLaunchKernelGGL(k_beeman, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, step > 0 ? 1 : 0,
                        step < (p.total_steps - 1) ? 1 : 0, mass.p, fbx.p, fby.p, fbz.p, ax, ay, az, orig_slot.p,
                        x.p, y.p, z.p, vx.p, vy.p, vz.p, fox.p, foy.p, foz.p, e_partial_ke.p);
-  } else if (p.integrator == PDGPU_VELOCITY_VERLET || gammat < 0.00001) {
-    // Langevin with gamma*dt < 1e-5 falls back to velocity Verlet (md.cpp:966-971)
+  } else if (vv) {
     hipLaunchKernelGGL(k_velocity_verlet, dim3(grid), dim3(T), 0, stream, n, p.timestep, conv, mass.p, fbx.p,
                        fby.p, fbz.p, ax, ay, az, orig_slot.p, x.p, y.p, z.p, vx.p, vy.p, vz.p, e_partial_ke.p);
   } else {
@@ -445,6 +722,20 @@ void Engine::integrate(const pdgpu_md_params &p, int step, double *ke_slot) {
   }
   PDGPU_HIP(hipGetLastError());
   defer_sum4(e_partial_ke.p, grid, ke_slot);
+}
+
+// calcTotalLinearMomentum .. zeroTotalAngularMomentum (md.cpp:263-272, 412-517).
+void Engine::remove_momentum() {
+  const int T = 256;
+  const int grid = (n + T - 1) / T;
+  mom_part.resize((size_t)grid * 10);
+  mom.resize(10);
+  hipLaunchKernelGGL(k_mom_partials, dim3(grid), dim3(T), 0, stream, n, mass.p, x.p, y.p, z.p, vx.p, vy.p, vz.p,
+                     mom_part.p);
+  hipLaunchKernelGGL(k_mom_finish, dim3(1), dim3(256), 0, stream, mom_part.p, grid, mom.p);
+  hipLaunchKernelGGL(k_mom_apply, dim3(grid), dim3(T), 0, stream, n, mass.p, x.p, y.p, z.p, vx.p, vy.p, vz.p,
+                     mom.p);
+  PDGPU_HIP(hipGetLastError());
 }
 
 // c0 c1 c2 varr varv crv of applyForces_LangevinIntegration (md.cpp:947-986).
@@ -501,7 +792,10 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
       build_list();
       evaluate(comp, step, false);
     }
-    integrate(p, step, e_out.p + 16);
+    integrate(p, step, e_out.p + 20);
+    // every(Step, UpdateRemoveTotalMomentum) with a thermostat (md.cpp:263-272, protocolbase.cpp:251-257)
+    if (p.thermostat != PDGPU_NO_THERMOSTAT && p.remove_momentum > 0 && step > 0 && step % p.remove_momentum == 0)
+      remove_momentum();
     const StepRecord rec{trace.p, s, (comp & PDGPU_NONBONDED) ? 1 : 0, (comp & PDGPU_BONDED) ? 1 : 0,
                          (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor, Consts::kcal_to_J()};
     flush_sums(&rec);

@@ -23,6 +23,9 @@ PDGPU_ENERGY_ONLY = 0x100
 PDGPU_VELOCITY_VERLET = 1
 PDGPU_BEEMAN = 2
 PDGPU_LANGEVIN = 3
+PDGPU_NO_THERMOSTAT = 0
+PDGPU_ANDERSEN = 1
+PDGPU_BERENDSEN = 2
 
 ERRORS = {-1: "argument", -2: "hip", -3: "setup", -4: "nomem", -5: "unstable"}
 
@@ -89,7 +92,8 @@ class MDParams(C.Structure):
     _fields_ = [("integrator", C.c_int32), ("components", C.c_uint32), ("timestep", C.c_double),
                 ("temperature", C.c_double), ("fric_coeff", C.c_double), ("update_nlist", C.c_int32),
                 ("step_offset", C.c_int32), ("total_steps", C.c_int32), ("seed", C.c_uint64),
-                ("noise_stream", C.c_uint32)]
+                ("noise_stream", C.c_uint32), ("thermostat", C.c_int32), ("remove_momentum", C.c_int32),
+                ("berendsen_tau", C.c_double), ("andersen_rate", C.c_double)]
 
 
 def nb_params(**kw) -> NBParams:

@@ -108,6 +108,57 @@ def test_md_recovers_from_list_overflow_at_a_rebuild():
     assert np.array_equal(ea, eb) and np.array_equal(ka, kb)
 
 
+def _thermo_engine():
+    s = S.water_box(512, seed=6)
+    e = G.engine_for(s, G.nb_params(**WATER_NB), padding=2.0, bonded=True)
+    e.build_list()
+    v0 = mb_velocities(s, 300.0, 3)
+    e.set_velocities(v0)
+    return s, e, v0
+
+
+@pytest.mark.parametrize("integ,thermo", [("vv", "berendsen"), ("vv", "andersen"), ("beeman", "berendsen"),
+                                          ("beeman", "andersen")])
+def test_thermostat_steps_match_oracle(integ, thermo):
+    """Two steps (step 1 of each protocol kicks, thermostats, moves) with the
+    momentum removal of run_core at step 1, against the oracle composed as PD's
+    step: kick, kinetic energy, applyThermostat, move; then the momentum step."""
+    s, e, v0 = _thermo_engine()
+    comps = G.PDGPU_NONBONDED | G.PDGPU_BONDED
+    I = G.PDGPU_VELOCITY_VERLET if integ == "vv" else G.PDGPU_BEEMAN
+    T = G.PDGPU_BERENDSEN if thermo == "berendsen" else G.PDGPU_ANDERSEN
+    rate = 0.3 * s.n ** (2.0 / 3.0)  # 30% of the atoms reassigned per step at 1 fs
+    seed, stream, dt, tau = 0xABCDEF, 4, 1e-15, 50e-15
+    pos = e.get_positions()
+    vel = v0.copy()
+    fold = np.zeros_like(pos)
+    for st in range(2):
+        print(f"case {integ} {thermo} step {st}", flush=True)
+        f, _ = e.compute(comps)
+        md = G.MDParams(integrator=I, components=comps, timestep=dt, temperature=280.0, update_nlist=10,
+                        step_offset=st, total_steps=3, seed=seed, noise_stream=stream, thermostat=T,
+                        remove_momentum=1, berendsen_tau=tau, andersen_rate=rate)
+        ep, ek = e.md(md, 1)
+        if integ == "vv":
+            O.vv_kick(s.mass, dt, f, vel)
+        else:
+            O.beeman_kick(s.mass, dt, st, f, vel)
+        K = O.kinetic_energy(s.mass, vel)
+        if thermo == "berendsen":
+            O.berendsen(K, 280.0, dt, tau, vel)
+        else:
+            assert O.andersen(s.mass, 280.0, rate, dt, seed, stream, st, vel) > 0
+        if integ == "vv":
+            O.vv_move(s.mass, dt, f, pos, vel)
+        else:
+            O.beeman_move(s.mass, dt, st, 3, f, fold, pos, vel)
+        if st > 0:
+            O.zero_momentum(s.mass, pos, vel)
+        assert ek[0] == pytest.approx(K, rel=1e-13)
+        np.testing.assert_allclose(e.get_positions(), pos, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(e.get_velocities(), vel, rtol=1e-11, atol=1e-11 * np.abs(vel).max())
+
+
 def test_langevin_step_matches_oracle():
     s = S.peptide_chain(10, seed=2)
     e = G.engine_for(s, None, bonded=True)

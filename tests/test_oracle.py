@@ -714,3 +714,95 @@ def test_cell_list_equals_all_pairs_list(case):
     assert p1.size > 0
     np.testing.assert_array_equal(c1, c2)
     np.testing.assert_array_equal(p1, p2)
+
+
+def test_berendsen_relaxes_the_kinetic_energy_by_dt_over_tau():
+    """chi^2 = 1 + dt/tau (Edes/K - 1) makes the kinetic energy after the scaling
+    K' = K + (dt/tau)(Edes - K), with Edes = 3 N kB T / 2."""
+    rng = np.random.default_rng(3)
+    n = 500
+    mass = rng.uniform(1e-27, 3e-26, n)
+    vel = rng.normal(size=(n, 3)) * 900.0
+    K = O.kinetic_energy(mass, vel)
+    T, dt, tau = 300.0, 1e-15, 100e-15
+    chi = O.berendsen(K, T, dt, tau, vel)
+    Edes = 1.5 * n * U.KB * T
+    assert O.kinetic_energy(mass, vel) == pytest.approx(K + dt / tau * (Edes - K), rel=1e-12)
+    assert chi == pytest.approx(math.sqrt(1 + dt / tau * (Edes / K - 1)), rel=1e-15)
+
+
+def test_andersen_reassigns_the_expected_fraction_with_maxwell_velocities():
+    """rate = AndersenRate dt 1e15 / N^(2/3) picks atoms with (rand % 10000) <= 10000 rate;
+    the new velocities are N(0, kT/m) per component (cut at 4 sigma) and
+    uncorrelated, with vy taken from the second Box-Muller pair."""
+    n = 40000
+    mass = np.full(n, 2.988e-26)  # water-like
+    vel = np.zeros((n, 3))
+    rate_param = 0.5 * n ** (2.0 / 3.0) / 1.0  # rate = 0.5 at dt = 1 fs
+    T = 300.0
+    changed = O.andersen(mass, T, rate_param, 1e-15, 77, 5, 9, vel)
+    frac = changed / n
+    assert abs(frac - 0.5) < 0.01
+    moved = np.abs(vel).sum(axis=1) > 0
+    assert moved.sum() == changed
+    sig = math.sqrt(U.KB * T / mass[0])
+    v = vel[moved] / sig
+    for d in range(3):
+        assert abs(v[:, d].mean()) < 0.03
+        assert abs(v[:, d].var() - 1.0) < 0.04
+    c = np.corrcoef(v.T)
+    assert np.abs(c - np.eye(3)).max() < 0.03
+    assert np.abs(v).max() <= 4.0
+
+
+def test_andersen_draws_are_uniform_and_distinct_per_atom():
+    t = np.array([O.andersen_draws(1, 2, 3, a)[0] for a in range(20000)])
+    assert t.min() >= 0 and t.max() < 10000
+    assert abs(t.mean() - 4999.5) < 60
+    z = np.array([O.andersen_draws(1, 2, 3, a)[1] for a in range(2000)])
+    assert z.min() > 0 and z.max() <= 1.0
+    assert abs(z.mean() - 0.5) < 0.02
+
+
+def test_zero_linear_momentum_leaves_minus_p_over_n_minus_1():
+    """zeroTotalLinearMomentum divides by (iend - istart) = N - 1, so the total
+    momentum afterwards is P - N P/(N-1) = -P/(N-1) (angular step aside: a
+    system at rest in rotation is used, so it changes nothing)."""
+    rng = np.random.default_rng(5)
+    n = 64
+    mass = rng.uniform(1e-27, 3e-26, n)
+    pos = np.zeros((n, 3))
+    pos[:, 0] = np.linspace(-5, 5, n)  # on a line: v along the line gives no angular momentum
+    vel = np.zeros((n, 3))
+    vel[:, 0] = rng.normal(size=n) * 500.0
+    P0 = (mass[:, None] * vel).sum(axis=0)
+    lin, ang = O.zero_momentum(mass, pos, vel)
+    np.testing.assert_allclose(lin, P0, rtol=1e-14)
+    np.testing.assert_allclose(ang, 0.0, atol=1e-40)
+    P1 = (mass[:, None] * vel).sum(axis=0)
+    np.testing.assert_allclose(P1, -P0 / (n - 1), rtol=1e-9, atol=1e-12 * np.abs(P0).max())
+
+
+def test_zero_angular_momentum_slows_a_rigid_rotation():
+    """For a rigid rotation about the centre of mass, every correction is
+    perpendicular to L and to the atom's lever arm, has magnitude
+    |L| / (|r| (N-1) m), and points along the motion, so |L| drops."""
+    rng = np.random.default_rng(6)
+    n = 50
+    mass = rng.uniform(1e-27, 3e-26, n)
+    pos = rng.normal(size=(n, 3)) * 4.0
+    com = (mass[:, None] * pos).sum(axis=0) / mass.sum()
+    w = np.array([0.0, 0.0, 2.0e12])  # rad/s about z
+    r = (pos - com) * 1e-10
+    vel = np.cross(w, r)
+    v0 = vel.copy()
+    lin, L = O.zero_momentum(mass, pos, vel)
+    dv = v0 - vel - lin / ((n - 1) * mass[:, None])  # the angular correction alone
+    Lm = np.linalg.norm(L)
+    for i in range(n):
+        assert abs(dv[i] @ L) <= 1e-9 * np.linalg.norm(dv[i]) * Lm
+        assert abs(dv[i] @ r[i]) <= 1e-9 * np.linalg.norm(dv[i]) * np.linalg.norm(r[i])
+        assert np.linalg.norm(dv[i]) == pytest.approx(Lm / (np.linalg.norm(r[i]) * (n - 1) * mass[i]), rel=1e-9)
+        assert dv[i] @ v0[i] >= 0.0
+    L1 = (mass[:, None] * np.cross(r, vel)).sum(axis=0)
+    assert np.linalg.norm(L1) < Lm
