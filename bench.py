@@ -172,6 +172,47 @@ def gb_engine(G, s, seed):
     return e
 
 
+def gb_cpu_baseline(G, s, e, comps):
+    """The FP64 oracle on the chain's final positions: PD's list (all pairs, shadow),
+    Born radii, the FastMode GB pass, bonded terms and LCPO, one thread; and the
+    engine's forces and energy at the same positions against it."""
+    from oracle import oracle as O
+    from pd_amd import units as U
+    from pd_amd import systems as S
+    s2 = s.copy()
+    s2.pos = e.get_positions()
+    p = O.gb_params(**GB_KW)
+    bo = O.bond_order(s2)
+    t0 = time.perf_counter()
+    counts, packed = O.build_list(s2, s2.pos, max(p.Cutoff, p.VdwCutoff), 0.5, True, bo, cells=True)
+    t_list = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Vj, t123, _ = O.gb_fixed_terms(s2, p)
+    born = O.gb_radii(s2, s2.pos, Vj, t123, p, counts, packed)
+    f_gb, e_gb = O.gb_forces_fast(s2, s2.pos, Vj, born, p, counts, packed)
+    f_b, e_b = O.bonded_forces(s2, s2.pos)
+    t_gb = time.perf_counter() - t0
+    use, rad, P = S.lcpo_params(s2)
+    sig = np.full(s2.n, 0.005 / U.NA / U.J2KCAL)
+    t0 = time.perf_counter()
+    f_l, _, e_l = O.lcpo_forces(s2.pos, use, rad, P, sig, counts, packed)
+    t_lcpo = time.perf_counter() - t0
+    per_step = t_list / 10.0 + t_gb + t_lcpo / 5.0
+    gf, ge = e.compute(comps)
+    fo = f_gb + f_b + f_l
+    eo = e_gb["epot"] + e_b["epot"] + e_l
+    return {
+        "value": round(1.0 / per_step, 3), "unit": "steps/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle on the same {s2.n}-atom chain, FP64, 1 thread: list {t_list:.2f} s (every 10 steps), "
+                   f"GB radii + FastMode pass + bonded {t_gb:.2f} s, LCPO {t_lcpo:.2f} s (every 5 steps)"),
+        "error": {
+            "force_rms_rel": float(np.sqrt(((gf - fo) ** 2).mean()) / np.sqrt((fo ** 2).mean())),
+            "energy_rel": float(abs(ge["epot"] - eo) / abs(eo)),
+            "reference": "FP64 oracle at the same positions (GB FastMode + bonded + LCPO)",
+        },
+    }
+
+
 def gb_benchmarks(args, log):
     """The GB/SA workloads: one 300-residue chain, 256 uneven chains batched vs the
     single-system engine looped over them, and 64 REMD replicas."""
@@ -211,6 +252,12 @@ def gb_benchmarks(args, log):
                              "algorithmic_flop_per_launch": pairs * GB_FLOP_PAIR},
     }
     log(f"single chain: {out['single_chain']['steps_per_s']} steps/s ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_cpu_baseline:
+        cb = gb_cpu_baseline(G, s, e, comps)
+        out["single_chain"]["error"] = cb.pop("error")
+        out["single_chain"]["cpu_baseline"] = cb
+        log(f"single chain on the CPU oracle: {cb['value']} steps/s; force rms rel "
+            f"{out['single_chain']['error']['force_rms_rel']:.2e}")
     del e
 
     # --- 256 uneven chains: batched vs looping the single-system engine --------
