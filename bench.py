@@ -130,9 +130,11 @@ def cpu_baseline(sysw, nb_kw, padding, eng=None, comps=0):
         gf, ge = eng.compute(comps)
         fo = f + fb
         eo = e_nb["epot"] + e_b["epot"]
+        terms = {k: e_nb[k] + e_b[k] for k in ("epot_vdw", "epot_elec", "epot_bond", "epot_angle")}
         out["error"] = {
             "force_rms_rel": float(np.sqrt(((gf - fo) ** 2).mean()) / np.sqrt((fo ** 2).mean())),
             "energy_rel": float(abs(ge["epot"] - eo) / abs(eo)),
+            "per_term_rel": {k: float(abs(ge[k] - v) / abs(v)) for k, v in terms.items() if v != 0.0},
             "epot_kcal_mol": round(eo * 6.02214076e23 / 4184.0, 3),
             "reference": "FP64 oracle at the same positions (nonbonded + bonded)",
         }
@@ -208,6 +210,10 @@ def gb_cpu_baseline(G, s, e, comps):
         "error": {
             "force_rms_rel": float(np.sqrt(((gf - fo) ** 2).mean()) / np.sqrt((fo ** 2).mean())),
             "energy_rel": float(abs(ge["epot"] - eo) / abs(eo)),
+            "per_term_rel": {k: float(abs(ge[k] - ref) / abs(ref)) for k, ref in (
+                ("epot_pol", e_gb["epot_pol"]), ("epot_vdw", e_gb["epot_vdw"]), ("epot_elec", e_gb["epot_elec"]),
+                ("epot_surf", e_l), ("epot_bond", e_b["epot_bond"]), ("epot_angle", e_b["epot_angle"]),
+                ("epot_torsion", e_b["epot_torsion"])) if ref != 0.0},
             "reference": "FP64 oracle at the same positions (GB FastMode + bonded + LCPO)",
         },
     }

@@ -254,10 +254,18 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
     const int a = in_lds ? sa[k] : col_atoms[s0 + k];
     const double za = in_lds ? sz[k] : wz[a];
     int rank = 0;
-    for (int m = 0; m < cnt; m++) {
-      const int b = in_lds ? sa[m] : col_atoms[s0 + m];
-      const double zb = in_lds ? sz[m] : wz[b];
-      rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
+    if (in_lds) {  // uniform: the common case runs from LDS, unrolled
+#pragma unroll 8
+      for (int m = 0; m < cnt; m++) {
+        const double zb = sz[m];
+        rank += (zb < za || (zb == za && sa[m] < a)) ? 1 : 0;
+      }
+    } else {
+      for (int m = 0; m < cnt; m++) {
+        const int b = col_atoms[s0 + m];
+        const double zb = wz[b];
+        rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
+      }
     }
     const int slot = col_cl_start[c] * kClusterSize + rank;
     slot_orig[slot] = a;
