@@ -27,6 +27,8 @@
 // per-wave FP64 partials reduced in a fixed order.
 #include "engine.hpp"
 
+#include <type_traits>
+
 #include <cmath>
 
 namespace pdgpu {
@@ -296,76 +298,85 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           }
         }
       }
-#pragma unroll
-      for (int a = 0; a < 8; a++) {
-        const float dx = dxa[a], dy = dya[a], dz = dza[a], r2s_a = r2a[a];
-        const bool inc = inca[a], inv = inva[a];
-        const float4 pa = (&iprm[0][0])[ib + a];
-        const bool on = ((lm >> a) & 1u) && !((excl >> (2 * a)) & 1u) && inc;
-        const bool s14 = (is14 >> (2 * a)) & 1u;
-        const float r2f = on ? r2s_a : 1.0f;
-        const float rinv = __builtin_amdgcn_rsqf(r2f);
-        const float r = r2f * rinv;
-        const float rinv2 = rinv * rinv;
-        // A pair that is off has eps = qq = 0, so every energy and force term below
-        // comes out as (signed) zero without a select of its own.
-        float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
-        if (VDW) {
-          const float eps = (on && inv) ? (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z : 0.0f;
-          const float Rij = pa.y + qj.y;
-          const float s2 = Rij * Rij * rinv2;
-          const float B = s2 * s2 * s2;
-          const float A = B * B;
-          float e0 = eps * (A - 2.0f * B);
-          float d0 = -12.0f * eps * rinv * (A - B);
-          const float t = fmaxf(r - P.vdwinner, 0.0f);
-          const float S0 = 1.0f - (P.invw_vdw * t) * (P.invw_vdw * t);
-          const float dS = -P.invw2_vdw4 * t * S0;
-          const float S = S0 * S0;
-          d0 = S * d0 + e0 * dS;
-          Ev = e0 * S;
-          dEdr += d0;
-        }
-        if (ELEC != EL_NONE) {
-          const float qq = on ? (s14 ? P.elec14 : 1.0f) * pa.x * qj.x : 0.0f;
-          if (ELEC == EL_PLAIN) {
-            Ee = qq * rinv;
-            dEdr += -Ee * rinv;
-          } else if (ELEC == EL_ESWITCH) {
-            float e0 = qq * rinv;
-            float d0 = -e0 * rinv;
-            const float t = fmaxf(r - P.inner, 0.0f);
-            const float S0 = 1.0f - (P.invw_el * t) * (P.invw_el * t);
-            const float dS = -P.invw2_el4 * t * S0;
+      // The pair loop in two versions: rounds whose entries carry no bond-order
+      // code (no 1-2, 1-3 or 1-4 pair; most rounds) skip decoding the codes.
+      auto pair_loop = [&](auto has_codes) {
+        constexpr bool HC = decltype(has_codes)::value;
+  #pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const float dx = dxa[a], dy = dya[a], dz = dza[a], r2s_a = r2a[a];
+          const bool inc = inca[a], inv = inva[a];
+          const float4 pa = (&iprm[0][0])[ib + a];
+          const bool on = ((lm >> a) & 1u) && (!HC || !((excl >> (2 * a)) & 1u)) && inc;
+          const bool s14 = HC && ((is14 >> (2 * a)) & 1u);
+          const float r2f = on ? r2s_a : 1.0f;
+          const float rinv = __builtin_amdgcn_rsqf(r2f);
+          const float r = r2f * rinv;
+          const float rinv2 = rinv * rinv;
+          // A pair that is off has eps = qq = 0, so every energy and force term below
+          // comes out as (signed) zero without a select of its own.
+          float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
+          if (VDW) {
+            const float eps = (on && inv) ? (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z : 0.0f;
+            const float Rij = pa.y + qj.y;
+            const float s2 = Rij * Rij * rinv2;
+            const float B = s2 * s2 * s2;
+            const float A = B * B;
+            float e0 = eps * (A - 2.0f * B);
+            float d0 = -12.0f * eps * rinv * (A - B);
+            const float t = fmaxf(r - P.vdwinner, 0.0f);
+            const float S0 = 1.0f - (P.invw_vdw * t) * (P.invw_vdw * t);
+            const float dS = -P.invw2_vdw4 * t * S0;
             const float S = S0 * S0;
             d0 = S * d0 + e0 * dS;
-            Ee = e0 * S;
+            Ev = e0 * S;
             dEdr += d0;
-          } else {  // force switch (Steinbach & Brooks), nonbonded.cpp:798-819
-            const bool outer = r > P.inner;
-            const float dc = P.rc2 - r2f;
-            const float d_out = -qq * P.sA * rinv2 * dc * dc * (P.rc2 - 3.0f * P.rin2 + 2.0f * r2f);
-            const float e_out = -qq * (P.sA * r * (P.sB * rinv2 + P.sC + P.sD * r2f + P.sE * r2f * r2f) - P.fcut);
-            const float d_in = -qq * rinv2;
-            const float e_in = qq * (rinv - P.eshift);
-            Ee = outer ? e_out : e_in;
-            dEdr += outer ? d_out : d_in;
           }
+          if (ELEC != EL_NONE) {
+            const float qq = on ? (s14 ? P.elec14 : 1.0f) * pa.x * qj.x : 0.0f;
+            if (ELEC == EL_PLAIN) {
+              Ee = qq * rinv;
+              dEdr += -Ee * rinv;
+            } else if (ELEC == EL_ESWITCH) {
+              float e0 = qq * rinv;
+              float d0 = -e0 * rinv;
+              const float t = fmaxf(r - P.inner, 0.0f);
+              const float S0 = 1.0f - (P.invw_el * t) * (P.invw_el * t);
+              const float dS = -P.invw2_el4 * t * S0;
+              const float S = S0 * S0;
+              d0 = S * d0 + e0 * dS;
+              Ee = e0 * S;
+              dEdr += d0;
+            } else {  // force switch (Steinbach & Brooks), nonbonded.cpp:798-819
+              const bool outer = r > P.inner;
+              const float dc = P.rc2 - r2f;
+              const float d_out = -qq * P.sA * rinv2 * dc * dc * (P.rc2 - 3.0f * P.rin2 + 2.0f * r2f);
+              const float e_out = -qq * (P.sA * r * (P.sB * rinv2 + P.sC + P.sD * r2f + P.sE * r2f * r2f) - P.fcut);
+              const float d_in = -qq * rinv2;
+              const float e_in = qq * (rinv - P.eshift);
+              Ee = outer ? e_out : e_in;
+              dEdr += outer ? d_out : d_in;
+            }
+          }
+          if (FORCES) {
+            const float sc = dEdr * rinv;
+            const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
+            fix[a] += gx;
+            fiy[a] += gy;
+            fiz[a] += gz;
+            fjx -= gx;
+            fjy -= gy;
+            fjz -= gz;
+          }
+          ev8 += Ev;
+          ee8 += Ee;
+          vir8 += r * dEdr;
         }
-        if (FORCES) {
-          const float sc = dEdr * rinv;
-          const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
-          fix[a] += gx;
-          fiy[a] += gy;
-          fiz[a] += gz;
-          fjx -= gx;
-          fjy -= gy;
-          fjz -= gz;
-        }
-        ev8 += Ev;
-        ee8 += Ee;
-        vir8 += r * dEdr;
-      }
+      };
+      if (__ballot(codes != 0u) != 0ull)
+        pair_loop(std::true_type{});
+      else
+        pair_loop(std::false_type{});
       ev += (double)ev8;
       ee += (double)ee8;
       vir += (double)vir8;
