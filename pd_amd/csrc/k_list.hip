@@ -235,10 +235,11 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
                                                       const int *__restrict__ col_atom_start,
                                                       const int *__restrict__ col_cl_start,
                                                       const int *__restrict__ col_atoms, const double *__restrict__ wz,
-                                                      int *__restrict__ slot_orig, int *__restrict__ orig_slot,
-                                                      int *__restrict__ cl_col) {
+                                                      const float *__restrict__ lj_eps, int *__restrict__ slot_orig,
+                                                      int *__restrict__ orig_slot, int *__restrict__ cl_col) {
   __shared__ double sz[kSortLds];
   __shared__ int sa[kSortLds];
+  __shared__ int by_rank[kSortLds];
   const int c = blockIdx.x;
   if (c >= ncol) return;
   const int s0 = col_atom_start[c], cnt = col_count[c];
@@ -260,17 +261,39 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
         const double zb = sz[m];
         rank += (zb < za || (zb == za && sa[m] < a)) ? 1 : 0;
       }
+      by_rank[rank] = a;
     } else {
       for (int m = 0; m < cnt; m++) {
         const int b = col_atoms[s0 + m];
         const double zb = wz[b];
         rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
       }
+      const int slot = col_cl_start[c] * kClusterSize + rank;
+      slot_orig[slot] = a;
+      orig_slot[a] = slot;
+      if ((rank & (kClusterSize - 1)) == 0) cl_col[slot / kClusterSize] = c;
     }
-    const int slot = col_cl_start[c] * kClusterSize + rank;
+  }
+  if (!in_lds) return;  // uniform
+  __syncthreads();
+  // Inside each cluster of 8 consecutive ranks, the atoms with a Lennard-Jones
+  // epsilon come first (in z order), then the others: the nonbonded kernel skips
+  // the LJ arithmetic for slots that hold none.
+  for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+    const int base = r & ~(kClusterSize - 1), end = min(base + kClusterSize, cnt);
+    const int a = by_rank[r];
+    const bool lj = lj_eps == nullptr || lj_eps[a] != 0.0f;
+    int pos = 0, nlj = 0;
+    for (int q = base; q < end; q++) {
+      const bool lq = lj_eps == nullptr || lj_eps[by_rank[q]] != 0.0f;
+      nlj += lq ? 1 : 0;
+      if (q < r && lq == lj) pos++;
+    }
+    if (!lj) pos += nlj;
+    const int slot = col_cl_start[c] * kClusterSize + base + pos;
     slot_orig[slot] = a;
     orig_slot[a] = slot;
-    if ((rank & (kClusterSize - 1)) == 0) cl_col[slot / kClusterSize] = c;
+    if (r == base) cl_col[slot / kClusterSize] = c;
   }
 }
 
@@ -724,7 +747,8 @@ void Engine::build_list(bool deferred) {
   const int nslots = max_clusters * kClusterSize;
   hipLaunchKernelGGL(k_fill_int, dim3((nslots + T - 1) / T), dim3(T), 0, stream, slot_orig.p, -1, nslots);
   hipLaunchKernelGGL(k_sort_columns, dim3(ncol), dim3(256), 0, stream, ncol, col_count.p, col_atom_start.p,
-                     col_cl_start.p, col_atoms.p, wz.p, slot_orig.p, orig_slot.p, cl_col.p);
+                     col_cl_start.p, col_atoms.p, wz.p, (have_nb && lj_e.n >= (size_t)n) ? lj_e.p : nullptr,
+                     slot_orig.p, orig_slot.p, cl_col.p);
   slot_list.resize((size_t)nslots * sizeof(SlotList));
   hipLaunchKernelGGL(k_slot_pack, dim3((nslots + T - 1) / T), dim3(T), 0, stream, nslots, n_clusters_d.p,
                      slot_orig.p, wx.p, wy.p, wz.p, atom_wrap.p, sp_off.p,

@@ -234,6 +234,10 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       ioff[wave][lane][2] = p.z - oi.z;
       iprm[wave][lane] = sprm[ci * kClusterSize + lane];
     }
+    // The column sort puts a cluster's atoms with an LJ epsilon first; a cluster
+    // whose slots 4..7 hold none (water: its hydrogens) skips LJ for them.
+    const bool lj4 =
+        (__ballot(lane >= 4 && lane < kClusterSize && sprm[ci * kClusterSize + lane].z != 0.0f) == 0ull);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -300,8 +304,9 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       }
       // The pair loop in two versions: rounds whose entries carry no bond-order
       // code (no 1-2, 1-3 or 1-4 pair; most rounds) skip decoding the codes.
-      auto pair_loop = [&](auto has_codes) {
+      auto pair_loop = [&](auto has_codes, auto lj_first4) {
         constexpr bool HC = decltype(has_codes)::value;
+        constexpr bool L4 = decltype(lj_first4)::value;
   #pragma unroll
         for (int a = 0; a < 8; a++) {
           const float dx = dxa[a], dy = dya[a], dz = dza[a], r2s_a = r2a[a];
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           // A pair that is off has eps = qq = 0, so every energy and force term below
           // comes out as (signed) zero without a select of its own.
           float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
-          if (VDW) {
+          if (VDW && (!L4 || a < 4)) {
             const float eps = (on && inv) ? (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z : 0.0f;
             const float Rij = pa.y + qj.y;
             const float s2 = Rij * Rij * rinv2;
@@ -373,10 +378,15 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           vir8 += r * dEdr;
         }
       };
-      if (__ballot(codes != 0u) != 0ull)
-        pair_loop(std::true_type{});
+      const bool anycodes = __ballot(codes != 0u) != 0ull;
+      if (anycodes && lj4)
+        pair_loop(std::true_type{}, std::true_type{});
+      else if (anycodes)
+        pair_loop(std::true_type{}, std::false_type{});
+      else if (lj4)
+        pair_loop(std::false_type{}, std::true_type{});
       else
-        pair_loop(std::false_type{});
+        pair_loop(std::false_type{}, std::false_type{});
       ev += (double)ev8;
       ee += (double)ee8;
       vir += (double)vir8;
