@@ -307,7 +307,7 @@ struct Engine {
   DBuf<uint2> pentries;
   DBuf<int> pcount;
   DBuf<SlotPos> pref;
-  DBuf<unsigned> prune_dmax;               // [2]: max displacement from pref (float bits), alternating per step
+  DBuf<unsigned> prune_dmax;               // max displacement from pref (float bits): 2 alternating sets of spread words
   int dmax_cur = 0;                        // which prune_dmax word this evaluation fills
   double prune_margin = 0.5;
   bool prune_forced = true;

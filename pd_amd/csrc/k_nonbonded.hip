@@ -40,6 +40,10 @@ enum { EL_NONE = 0, EL_PLAIN = 1, EL_ESWITCH = 3, EL_FSWITCH = 4 };
 // |r^2_fp32 - r^2_exact| is below 1e-5 r^2 (one FP32 rounding of d, three of the
 // sum); pairs within this band of a cutoff are re-tested in FP64.
 constexpr float kNearBand = 0.01f;
+// The largest displacement since the last prune is kept as kDmaxSpread words
+// (float bits, max-combined) kDmaxStride words apart, two sets alternating by step.
+constexpr int kDmaxSpread = 64;
+constexpr int kDmaxStride = 16;
 
 // Image index (i+1)*9 + (j+1)*3 + (k+1) -> the basis vector (i*Lx, j*Ly, k*Lz)
 // of PeriodicBox::setupCellDimensions (space.cpp:94-109).
@@ -50,8 +54,8 @@ __device__ __forceinline__ double image_shift(int img, int axis, double L) {
 
 // First kernel of an evaluation with the nonbonded term.  Gathers the FP64
 // positions into slot order, folds each atom's displacement since the last
-// prune into dmax[cur] (one atomic per wave; dmax[1 - cur] is cleared for the
-// next evaluation) and clears the force accumulators and energy slots.
+// prune into this evaluation's set of spread maxima (the other set is cleared
+// for the next evaluation) and clears the force accumulators and energy slots.
 __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
                            const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                            SlotPos *__restrict__ spos, const SlotPos *__restrict__ pref, unsigned *__restrict__ dmax,
@@ -59,7 +63,7 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
                            long long *__restrict__ nbz, long long *__restrict__ fbx, long long *__restrict__ fby,
                            long long *__restrict__ fbz, double *__restrict__ e_out, int n_e) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0 && dmax) dmax[1 - cur] = 0u;
+  if (dmax && s < kDmaxSpread) dmax[((1 - cur) * kDmaxSpread + s) * kDmaxStride] = 0u;
   if (s < n_e) e_out[s] = 0.0;
   if (s < n) {
     fbx[s] = 0;
@@ -93,7 +97,11 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
   }
   d = wave_max(d);
   // rounded up (float bits of non-negative values order like the values)
-  if ((threadIdx.x & 63) == 0 && d > 0.0) atomicMax(&dmax[cur], __float_as_uint((float)(d * (1.0 + 1e-6))));
+  // one atomic per wave, spread over kDmaxSpread words on separate cache lines:
+  // device-scope atomics on one address serialise
+  if ((threadIdx.x & 63) == 0 && d > 0.0)
+    atomicMax(&dmax[(cur * kDmaxSpread + (blockIdx.x % kDmaxSpread)) * kDmaxStride],
+              __float_as_uint((float)(d * (1.0 + 1e-6))));
 }
 
 // Rolling prune of the list: keeps the pairs within Cutoff + margin at the
@@ -107,13 +115,21 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
 constexpr int kPruneDepth = 4;
 __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
                                                float rk2, float margin_ok, int forced,
-                                               const unsigned *__restrict__ dmax,  // this evaluation's word
+                                               const unsigned *__restrict__ dmax,  // this evaluation's words
                                                const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
                                                const int *__restrict__ cl_count, int stride,
                                                uint2 *__restrict__ pentries, int *__restrict__ pcount,
                                                SlotPos *__restrict__ pref) {
   __shared__ double shtab[27 * 3];
-  if (!forced && 2.0f * __uint_as_float(*dmax) <= margin_ok) return;
+  __shared__ unsigned s_dmax;
+  if (threadIdx.x < 64) {  // the largest of the spread words
+    unsigned v = threadIdx.x < kDmaxSpread ? dmax[threadIdx.x * kDmaxStride] : 0u;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+    if (threadIdx.x == 0) s_dmax = v;
+  }
+  __syncthreads();
+  if (!forced && 2.0f * __uint_as_float(s_dmax) <= margin_ok) return;
   if (threadIdx.x < 81) {
     const int axis = threadIdx.x % 3;
     shtab[threadIdx.x] = image_shift(threadIdx.x / 3, axis, axis == 0 ? Lx : (axis == 1 ? Ly : Lz));
@@ -208,9 +224,8 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
   const int g = (per * 8 == nblk) ? (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int wid = g * kGroup + wave;  // energy partial slot of this wave
   if (g >= *n_groups) {  // the whole workgroup leaves together
-    if (lane < 4) e_partial[4 * wid + lane] = 0.0;
+    if (threadIdx.x < 4) e_partial[4 * g + threadIdx.x] = 0.0;
     return;
   }
   const int nr = wg_ncand[g];
@@ -418,16 +433,23 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       }
     }
   }
+  // one energy partial per workgroup: the waves' sums in wave order
+  __shared__ double s_e[kGroup][3];
   const double sev = wave_sum(ev);
   const double see = wave_sum(ee);
   const double svir = wave_sum(vir);
   if (lane == 0) {
-    e_partial[4 * wid + 0] = sev;
-    e_partial[4 * wid + 1] = see;
-    e_partial[4 * wid + 2] = svir;
-    e_partial[4 * wid + 3] = 0.0;
+    s_e[wave][0] = sev;
+    s_e[wave][1] = see;
+    s_e[wave][2] = svir;
   }
   __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    if (threadIdx.x < 3)
+      for (int w = 0; w < kGroup; w++) t += s_e[w][threadIdx.x];
+    e_partial[4 * g + threadIdx.x] = t;
+  }
   // flush the touched slots of the candidate table, consecutive threads on
   // consecutive slots of a j-cluster
   const int ns = nr * kClusterSize;
@@ -558,9 +580,10 @@ void Engine::begin_evaluation(bool nonbonded) {
   const int nslots = max_clusters * kClusterSize;
   spos.resize(nslots);
   pref.resize(nslots);
-  if (prune_dmax.n < 2) {
-    prune_dmax.resize(2);
-    PDGPU_HIP(hipMemsetAsync(prune_dmax.p, 0, 2 * sizeof(unsigned), stream));
+  const size_t ndm = 2 * kDmaxSpread * kDmaxStride;
+  if (prune_dmax.n < ndm) {
+    prune_dmax.resize(ndm);
+    PDGPU_HIP(hipMemsetAsync(prune_dmax.p, 0, ndm * sizeof(unsigned), stream));
   }
   dmax_cur ^= 1;
   const int m = std::max(std::max(nslots, n), 24);
@@ -591,12 +614,11 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
     hipLaunchKernelGGL(k_prune, dim3((max_clusters + 3) / 4), dim3(256), 0, stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
-                       (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur, spos.p,
+                       (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur * kDmaxSpread * kDmaxStride, spos.p,
                        entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
     prune_forced = false;
   }
-  const int waves = max_groups * kGroup;
-  e_partial.resize((size_t)waves * 4);
+  e_partial.resize((size_t)max_groups * 4);
   const bool v = nbk.vdw_on != 0;
   {
     ScopedTimer t(*this, "nonbonded");
@@ -608,7 +630,7 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     }
   }
   PDGPU_HIP(hipGetLastError());
-  defer_sum4(e_partial.p, waves, energy_out);
+  defer_sum4(e_partial.p, max_groups, energy_out);
 }
 
 }  // namespace pdgpu
