@@ -15,9 +15,10 @@ namespace pdgpu {
 
 namespace {
 
-// Sums of 4-wide energy partials, one list entry after the other, by one
-// workgroup in a fixed order; then, for an MD step, the step's trace entry:
-// trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
+// Sums of 4-wide energy partials, all lists at once, by one workgroup; each
+// list is summed in a fixed order (thread-strided partial sums, then each wave,
+// then the 16 wave sums in wave order). Then, for an MD step, the step's trace
+// entry: trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
 // per component slot; the kinetic energy slot is in J).
 constexpr int kMaxSums = 8;
 struct SumList {
@@ -31,36 +32,54 @@ struct SumList {
 };
 
 __global__ __launch_bounds__(1024) void k_sums(SumList L) {
-  __shared__ double s[16][4];
+  __shared__ double s[16][kMaxSums][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int e = 0; e < L.n; e++) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
-    for (int i = threadIdx.x; i < L.count[e]; i += 1024) {
-      const double4 v = P[i];
-      a0 += v.x;
-      a1 += v.y;
-      a2 += v.z;
-      a3 += v.w;
-    }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    a2 = wave_sum(a2);
-    a3 = wave_sum(a3);
-    if (lane == 0) {
-      s[w][0] = a0;
-      s[w][1] = a1;
-      s[w][2] = a2;
-      s[w][3] = a3;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-      double t = 0.0;
-      for (int k = 0; k < 16; k++) t += s[k][threadIdx.x];
-      L.out[e][threadIdx.x] = t;
-    }
-    __syncthreads();
+  // each thread issues its first load of every list before any use, so the
+  // lists' global loads are in flight together; longer lists continue in a loop
+  double4 first[kMaxSums];
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    first[e] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (e < L.n && (int)threadIdx.x < L.count[e])
+      first[e] = reinterpret_cast<const double4 *>(L.part[e])[threadIdx.x];
   }
+  double a[kMaxSums][4];
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    a[e][0] = first[e].x;
+    a[e][1] = first[e].y;
+    a[e][2] = first[e].z;
+    a[e][3] = first[e].w;
+    if (e < L.n) {
+      const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
+      for (int i = threadIdx.x + 1024; i < L.count[e]; i += 1024) {
+        const double4 v = P[i];
+        a[e][0] += v.x;
+        a[e][1] += v.y;
+        a[e][2] += v.z;
+        a[e][3] += v.w;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    if (e < L.n) {
+      for (int d = 0; d < 4; d++) {
+        const double t = wave_sum(a[e][d]);
+        if (lane == 0) s[w][e][d] = t;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 * kMaxSums) {
+    const int e = threadIdx.x >> 2, d = threadIdx.x & 3;
+    if (e < L.n) {
+      double t = 0.0;
+      for (int k = 0; k < 16; k++) t += s[k][e][d];
+      L.out[e][d] = t;
+    }
+  }
+  __syncthreads();
   if (L.record && threadIdx.x == 0) {
     const double *E = L.e_out;
     const StepRecord &R = L.rec;
