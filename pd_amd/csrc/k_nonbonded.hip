@@ -215,7 +215,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
   __shared__ long long jacc[3][kGroupRanks * kClusterSize];
   __shared__ int cand[kGroupRanks];
   __shared__ double shtab[27 * 3];
-  __shared__ double ioff[kGroup][kClusterSize][4];  // i atoms: FP64 offset from the cluster's first atom
+  __shared__ float4 ioff[kGroup][kClusterSize];     // i atoms: FP32 offset from the cluster's first atom
   __shared__ float4 iprm[kGroup][kClusterSize];     // i atoms: q, LJ radius, sqrt(eps)
   // XCD-aware order: consecutive workgroups are spread round-robin over the 8
   // XCDs, so give each XCD a contiguous run of (spatially adjacent) groups.
@@ -244,9 +244,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
     const SlotPos oi = spos[ci * kClusterSize];  // the cluster's first atom (always a real atom)
     if (lane < kClusterSize) {
       const SlotPos p = spos[ci * kClusterSize + lane];
-      ioff[wave][lane][0] = p.x - oi.x;
-      ioff[wave][lane][1] = p.y - oi.y;
-      ioff[wave][lane][2] = p.z - oi.z;
+      ioff[wave][lane] = make_float4((float)(p.x - oi.x), (float)(p.y - oi.y), (float)(p.z - oi.z), 0.0f);
       iprm[wave][lane] = sprm[ci * kClusterSize + lane];
     }
     // The column sort puts a cluster's atoms with an LJ epsilon first; a cluster
@@ -276,38 +274,47 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       const unsigned clo = codes & 0x5555u, chi = (codes >> 1) & 0x5555u;
       const unsigned excl = clo ^ chi;  // codes 1 and 2: 1-2 / 1-3 pairs, scale 0
       const unsigned is14 = clo & chi;  // code 3: 1-4 pair
-      const SlotPos pj = spos[j];
       const float4 qj = sprm[j];
-      const double shx = shtab[3 * img], shy = shtab[3 * img + 1], shz = shtab[3 * img + 2];
-      const double ax = (pj.x - oi.x) + shx, ay = (pj.y - oi.y) + shy, az = (pj.z - oi.z) + shz;
-      float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
-      float ev8 = 0.0f, ee8 = 0.0f, vir8 = 0.0f;  // the 8 pairs of this entry
+      // the j atom's image relative to the cluster's first atom: FP64, then FP32;
+      // the pair displacements below are FP32 differences of FP32 offsets, within
+      // ~1e-6 A of the FP64 displacement (the cutoff band below is 0.01 A^2 wide)
+      float ax, ay, az;
+      {
+        const SlotPos pj = spos[j];
+        ax = (float)((pj.x - oi.x) + shtab[3 * img]);
+        ay = (float)((pj.y - oi.y) + shtab[3 * img + 1]);
+        az = (float)((pj.z - oi.z) + shtab[3 * img + 2]);
+      }
+      // -0.0f is the additive identity, so the first term of each sum needs no add
+      float fjx = -0.0f, fjy = -0.0f, fjz = -0.0f;
+      float ev8 = -0.0f, ee8 = -0.0f, vir8 = -0.0f;  // the 8 pairs of this entry
       // Re-read the i atoms from LDS in every round instead of holding 8 atoms in
       // registers: the empty asm hides that the index is loop-invariant.
       int ib = wave * kClusterSize;
       asm volatile("" : "+v"(ib));
-      // cutoff decisions for the 8 pairs first: FP32 r^2 from the FP64 relative
-      // displacement; a pair within kNearBand of a cutoff takes PD's FP64 test
-      // (nonbonded.cpp:718-722) behind one wave-uniform branch, rarely taken
-      float dxa[8], dya[8], dza[8], r2a[8];
+      // cutoff decisions for the 8 pairs first, in FP32; a pair within kNearBand
+      // of a cutoff takes PD's FP64 test (nonbonded.cpp:718-722) behind one
+      // wave-uniform branch, rarely taken (also for pairs the entry does not list:
+      // their decision is masked off later)
+      float r2a[8];
       bool inca[8], inva[8];
       bool anynear = false;
 #pragma unroll
       for (int a = 0; a < 8; a++) {
-        const double *io = &ioff[0][0][0] + 4 * (ib + a);
-        dxa[a] = (float)(ax - io[0]);
-        dya[a] = (float)(ay - io[1]);
-        dza[a] = (float)(az - io[2]);
-        r2a[a] = dxa[a] * dxa[a] + dya[a] * dya[a] + dza[a] * dza[a];
+        const float4 io = (&ioff[0][0])[ib + a];
+        const float dx = ax - io.x, dy = ay - io.y, dz = az - io.z;
+        r2a[a] = dx * dx + dy * dy + dz * dz;
         inca[a] = r2a[a] <= cut2f;
         inva[a] = r2a[a] < vcut2f;
-        anynear |= ((lm >> a) & 1u) && fminf(fabsf(r2a[a] - cut2f), fabsf(r2a[a] - vcut2f)) < kNearBand;
+        anynear |= fabsf(r2a[a] - cut2f) < kNearBand || fabsf(r2a[a] - vcut2f) < kNearBand;
       }
-      if (__builtin_expect(__ballot(anynear) != 0ull, 0)) {
+      if (__builtin_expect(__ballot(anynear && lm != 0u) != 0ull, 0)) {
+        const SlotPos pj = spos[j];
+        const double shx = shtab[3 * img], shy = shtab[3 * img + 1], shz = shtab[3 * img + 2];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
-          const bool near = ((lm >> a) & 1u) && fminf(fabsf(r2a[a] - cut2f), fabsf(r2a[a] - vcut2f)) < kNearBand;
-          if (near) {
+          const bool near = fabsf(r2a[a] - cut2f) < kNearBand || fabsf(r2a[a] - vcut2f) < kNearBand;
+          if (near && ((lm >> a) & 1u)) {
 #pragma clang fp contract(off)
             const SlotPos pi = spos[ci * kClusterSize + a];
             const double ex = (pj.x - pi.x) + shx, ey = (pj.y - pi.y) + shy, ez = (pj.z - pi.z) + shz;
@@ -324,7 +331,9 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
         constexpr bool L4 = decltype(lj_first4)::value;
   #pragma unroll
         for (int a = 0; a < 8; a++) {
-          const float dx = dxa[a], dy = dya[a], dz = dza[a], r2s_a = r2a[a];
+          // the displacement again (cheaper than keeping 24 registers live)
+          const float4 io = (&ioff[0][0])[ib + a];
+          const float dx = ax - io.x, dy = ay - io.y, dz = az - io.z, r2s_a = r2a[a];
           const bool inc = inca[a], inv = inva[a];
           const float4 pa = (&iprm[0][0])[ib + a];
           const bool on = ((lm >> a) & 1u) && (!HC || !((excl >> (2 * a)) & 1u)) && inc;
@@ -337,7 +346,10 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           // comes out as (signed) zero without a select of its own.
           float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
           if (VDW && (!L4 || a < 4)) {
-            const float eps = (on && inv) ? (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z : 0.0f;
+            const float eps_ij = (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z;
+            // computed for every pair, then selected: a select of a product that
+            // reads pa would be turned into a branch around the LDS read
+            const float eps = (on && inv) ? eps_ij : 0.0f;
             const float Rij = pa.y + qj.y;
             const float s2 = Rij * Rij * rinv2;
             const float B = s2 * s2 * s2;
@@ -353,7 +365,8 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
             dEdr += d0;
           }
           if (ELEC != EL_NONE) {
-            const float qq = on ? (s14 ? P.elec14 : 1.0f) * pa.x * qj.x : 0.0f;
+            const float qq_ij = (s14 ? P.elec14 : 1.0f) * pa.x * qj.x;
+            const float qq = on ? qq_ij : 0.0f;
             if (ELEC == EL_PLAIN) {
               Ee = qq * rinv;
               dEdr += -Ee * rinv;
@@ -380,17 +393,16 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           }
           if (FORCES) {
             const float sc = dEdr * rinv;
-            const float gx = dx * sc, gy = dy * sc, gz = dz * sc;
-            fix[a] += gx;
-            fiy[a] += gy;
-            fiz[a] += gz;
-            fjx -= gx;
-            fjy -= gy;
-            fjz -= gz;
+            fix[a] = fmaf(dx, sc, fix[a]);
+            fiy[a] = fmaf(dy, sc, fiy[a]);
+            fiz[a] = fmaf(dz, sc, fiz[a]);
+            fjx = fmaf(-dx, sc, fjx);
+            fjy = fmaf(-dy, sc, fjy);
+            fjz = fmaf(-dz, sc, fjz);
           }
           ev8 += Ev;
           ee8 += Ee;
-          vir8 += r * dEdr;
+          vir8 = fmaf(r, dEdr, vir8);
         }
       };
       const bool anycodes = __ballot(codes != 0u) != 0ull;
