@@ -25,6 +25,12 @@ namespace {
 
 constexpr int kMaxCand = 1536;  // candidate j-clusters per work group (LDS)
 constexpr int kCandPart = 512;  // candidates found by one wave of the group
+// The candidate table is ordered by the distance of each candidate's box from
+// the group's box, in three bands (below 4 A, below 8 A, the rest): the nearest
+// candidates hold the most list entries, and the first kGroupRanks of the table
+// get LDS accumulators in the nonbonded kernel.
+constexpr int kDistBands = 3;
+constexpr double kBand1 = 16.0, kBand2 = 64.0;  // squared box distances (A^2)
 
 __global__ void k_clean_space(int n_mol, const int *__restrict__ first, const int *__restrict__ last,
                               double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
@@ -410,6 +416,7 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   __shared__ unsigned char join_sh[kMaxCand];
   __shared__ SlotList s_i[kGroup][kClusterSize];
   __shared__ int part_n[kGroup];
+  __shared__ int part_nb[kGroup][kDistBands];
   __shared__ int s_ncand, s_over;
   int *const cand_cl = cand, *const cand_sh = cand + kGroup * kCandPart;
   const int g = blockIdx.x;
@@ -435,7 +442,8 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   // group's box.  Only j-clusters >= the group's first cluster are candidates
   // (each unordered cluster pair is searched from its lower cluster).  The
   // stencil rows are dealt to the waves round-robin; each wave fills its own
-  // part of the LDS table and the parts are joined in wave order.
+  // part of the LDS table with each candidate's distance band, and the parts
+  // are joined band by band, in wave order within a band.
   {
     const int kz_lo = A.periodic ? -1 : 0, kz_hi = A.periodic ? 1 : 0;
     int nw = 0;  // this wave's candidates
@@ -468,17 +476,20 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
             const int t = base + lane;
             const int cj = cs + t;
             bool ok = (t < nc) && (cj >= c0) && !(sh == 13 && cj < c0 + ng);
+            int band_id = 0;
             if (ok) {
               double g0 = fmax(0.0, fmax(cl_lo[3 * cj] + shx - ghi[0], glo[0] - (cl_hi[3 * cj] + shx)));
               double g1 = fmax(0.0, fmax(cl_lo[3 * cj + 1] + shy - ghi[1], glo[1] - (cl_hi[3 * cj + 1] + shy)));
               double g2 = fmax(0.0, fmax(cl_lo[3 * cj + 2] + shz - ghi[2], glo[2] - (cl_hi[3 * cj + 2] + shz)));
-              ok = (g0 * g0 + g1 * g1 + g2 * g2) <= A.Rbox2;
+              const double d2 = g0 * g0 + g1 * g1 + g2 * g2;
+              ok = d2 <= A.Rbox2;
+              band_id = (d2 > kBand1 ? 1 : 0) + (d2 > kBand2 ? 1 : 0);
             }
             const unsigned long long b = __ballot(ok);
             const int pre = __popcll(b & ((1ull << lane) - 1ull));
             if (ok && nw + pre < kCandPart) {
               my_cl[nw + pre] = cj;
-              my_sh[nw + pre] = sh;
+              my_sh[nw + pre] = sh | (band_id << 8);
             }
             nw += __popcll(b);
           }
@@ -486,6 +497,19 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
       }
     }
     if (lane == 0) part_n[wave] = nw;
+    // this wave's candidates per distance band (its part is read back by other lanes)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int m = min(nw, kCandPart);
+    int nb[kDistBands] = {};
+    for (int k0 = 0; k0 < m; k0 += 64) {
+      const int bid = k0 + lane < m ? (my_sh[k0 + lane] >> 8) : -1;
+#pragma unroll
+      for (int q = 0; q < kDistBands; q++) nb[q] += __popcll(__ballot(bid == q));
+    }
+    if (lane == 0)
+      for (int q = 0; q < kDistBands; q++) part_nb[wave][q] = nb[q];
   }
   __syncthreads();
   if (threadIdx.x == 0) {  // join: own clusters, then the waves' parts in order
@@ -499,14 +523,33 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   }
   __syncthreads();
   {
-    int off = ng;
-    for (int w = 0; w < wave; w++) off += min(part_n[w], kCandPart);
+    // this wave's start in each band: the own clusters, the nearer bands, then
+    // the preceding waves' candidates of the same band
+    int off[kDistBands];
+    int start = ng;
+#pragma unroll
+    for (int q = 0; q < kDistBands; q++) {
+      off[q] = start;
+      for (int w = 0; w < kGroup; w++) {
+        if (w < wave) off[q] += part_nb[w][q];
+        start += part_nb[w][q];
+      }
+    }
     const int m = min(part_n[wave], kCandPart);
-    // move this wave's part after the preceding ones (a separate join buffer keeps the reads and writes apart)
-    for (int k = lane; k < m; k += 64) {
-      if (off + k < kMaxCand) {
-        join_cl[off + k] = cand_cl[wave * kCandPart + k];
-        join_sh[off + k] = (unsigned char)cand_sh[wave * kCandPart + k];
+    // move this wave's part into place (a separate join buffer keeps the reads and writes apart)
+    for (int k0 = 0; k0 < m; k0 += 64) {
+      const int k = k0 + lane;
+      const int v = k < m ? cand_sh[wave * kCandPart + k] : -1;
+      const int bid = v >= 0 ? (v >> 8) : -1;
+#pragma unroll
+      for (int q = 0; q < kDistBands; q++) {
+        const unsigned long long bq = __ballot(bid == q);
+        const int pos = off[q] + __popcll(bq & ((1ull << lane) - 1ull));
+        if (bid == q && pos < kMaxCand) {
+          join_cl[pos] = cand_cl[wave * kCandPart + k];
+          join_sh[pos] = (unsigned char)(v & 0xFF);
+        }
+        off[q] += __popcll(bq);
       }
     }
     if (wave == 0 && lane < ng) {
