@@ -17,10 +17,11 @@ namespace {
 
 // Sums of 4-wide energy partials, all lists at once, by one workgroup; each
 // list is summed in a fixed order (thread-strided partial sums, then each wave,
-// then the 16 wave sums in wave order). Then, for an MD step, the step's trace
+// then the wave sums in wave order). Then, for an MD step, the step's trace
 // entry: trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
 // per component slot; the kinetic energy slot is in J).
 constexpr int kMaxSums = 8;
+constexpr int kSumThreads = 256;
 struct SumList {
   const double *part[kMaxSums];
   double *out[kMaxSums];
@@ -31,8 +32,10 @@ struct SumList {
   const double *e_out;
 };
 
-__global__ __launch_bounds__(1024) void k_sums(SumList L) {
-  __shared__ double s[16][kMaxSums][4];
+template <int T>
+__global__ __launch_bounds__(T) void k_sums(SumList L) {
+  constexpr int W = T / 64;
+  __shared__ double s[W][kMaxSums][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // each thread issues its first load of every list before any use, so the
   // lists' global loads are in flight together; longer lists continue in a loop
@@ -52,7 +55,20 @@ __global__ __launch_bounds__(1024) void k_sums(SumList L) {
     a[e][3] = first[e].w;
     if (e < L.n) {
       const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
-      for (int i = threadIdx.x + 1024; i < L.count[e]; i += 1024) {
+      int i = threadIdx.x + T;
+      for (; i + 3 * T < L.count[e]; i += 4 * T) {  // four loads in flight, added in order
+        double4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = P[i + u * T];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          a[e][0] += v[u].x;
+          a[e][1] += v[u].y;
+          a[e][2] += v[u].z;
+          a[e][3] += v[u].w;
+        }
+      }
+      for (; i < L.count[e]; i += T) {
         const double4 v = P[i];
         a[e][0] += v.x;
         a[e][1] += v.y;
@@ -75,7 +91,7 @@ __global__ __launch_bounds__(1024) void k_sums(SumList L) {
     const int e = threadIdx.x >> 2, d = threadIdx.x & 3;
     if (e < L.n) {
       double t = 0.0;
-      for (int k = 0; k < 16; k++) t += s[k][e][d];
+      for (int k = 0; k < W; k++) t += s[k][e][d];
       L.out[e][d] = t;
     }
   }
@@ -577,7 +593,7 @@ void Engine::flush_sums(const StepRecord *rec) {
     L.record = (last && rec) ? 1 : 0;
     if (rec) L.rec = *rec;
     L.e_out = e_out.p;
-    hipLaunchKernelGGL(k_sums, dim3(1), dim3(1024), 0, stream, L);
+    hipLaunchKernelGGL(k_sums<kSumThreads>, dim3(1), dim3(kSumThreads), 0, stream, L);
   } while (k < pending_sums.size());
   pending_sums.clear();
   PDGPU_HIP(hipGetLastError());

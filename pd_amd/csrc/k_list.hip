@@ -610,40 +610,67 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   // image to be implied, take PD's own arithmetic on its positions.
   uint2 *E = entries + (size_t)ci * A.stride;
   const int jj = lane & 7;
+  // A listed pair's image is PD's minimum image, which equals the candidate's
+  // shift plus the difference of the two atoms' wrap counts (the exact path
+  // below accepts a pair only then), so it is jimg - (the i atom's wrap image).
+  // When the cluster's atoms share one wrap image, every lane's pairs share one
+  // image and each lane writes at most one entry per round.
+  int wimg[kClusterSize];
+  bool i_uniform = true;
+#pragma unroll
+  for (int ii = 0; ii < kClusterSize; ii++) {
+    wimg[ii] = wrap_image(s_i[wave][ii].aux);
+    if (s_i[wave][ii].orig >= 0 && wimg[ii] != wrap_image(s_i[wave][0].aux)) i_uniform = false;
+  }
   int cnt = 0;
+  // the first round's j slot; each round loads the next round's while it tests
+  int nk = (lane >> 3) < nnear ? (int)near[lane >> 3] : -1;
+  SlotList nlj = sl[(nk >= 0 ? join_cl[nk] : ci) * kClusterSize + jj];
   for (int b0 = 0; b0 < nnear; b0 += 8) {
-    const int t = b0 + (lane >> 3);
-    const bool have = t < nnear;
-    const int k = have ? (int)near[t] : 0;
+    const bool have = nk >= 0;
+    const int k = have ? nk : 0;
     const int cj = have ? join_cl[k] : ci;
     const int csh = have ? (int)join_sh[k] : 13;
+    const SlotList lj = nlj;
+    {
+      const int t = b0 + 8 + (lane >> 3);
+      nk = t < nnear ? (int)near[t] : -1;
+      nlj = sl[(nk >= 0 ? join_cl[nk] : ci) * kClusterSize + jj];
+    }
     const int sx = csh / 9 - 1, sy = (csh / 3) % 3 - 1, sz = csh % 3 - 1;
     const double shx = sx * A.L[0], shy = sy * A.L[1], shz = sz * A.L[2];
     const int sb = cj * kClusterSize + jj;
-    const SlotList lj = sl[sb];
     const int ob = lj.orig;
     const double jx = lj.wx + shx, jy = lj.wy + shy, jz = lj.wz + shz;
     const int jimg = 13 + 9 * sx + 3 * sy + sz + wrap_image(lj.aux);
     unsigned lm = 0, codes = 0;
-    unsigned long long imgs = 0;  // 5 bits per i atom
     const bool jok = have && ob >= 0;
-#pragma unroll 1
+    // The 8 tests without divergent branches: pairs clearly inside or outside
+    // rlist are decided from the wrapped positions; the rare band pairs (every
+    // pair when the box is too small) take PD's arithmetic behind one
+    // wave-uniform branch, and so do the bond-order lookups.
+    unsigned amb = 0;  // pairs for PD's arithmetic
+#pragma unroll 4
     for (int ii = 0; ii < kClusterSize; ii++) {
       const SlotList li = s_i[wave][ii];
-      const int oa = li.orig;
-      bool in = jok && oa >= 0 && (cj != ci || ii < jj);
-      int img = jimg - wrap_image(li.aux);
-      if (in) {
-        const double dx = jx - li.wx, dy = jy - li.wy, dz = jz - li.wz;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (A.fast && r2 < A.rlist2_lo) {
-          in = true;
-        } else if (A.fast && r2 > A.rlist2_hi) {
-          in = false;
-        } else {
+      const bool cand_ok = jok && li.orig >= 0 && (cj != ci || ii < jj);
+      const double dx = jx - li.wx, dy = jy - li.wy, dz = jz - li.wz;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool in_fast = A.fast && r2 < A.rlist2_lo;
+      const bool undecided = !A.fast || r2 <= A.rlist2_hi;
+      lm |= (cand_ok && in_fast) ? 1u << ii : 0u;
+      amb |= (cand_ok && !in_fast && undecided) ? 1u << ii : 0u;
+    }
+    if (__ballot(amb != 0u)) {
+#pragma unroll 1
+      for (int ii = 0; ii < kClusterSize; ii++) {
+        if ((amb >> ii) & 1u) {
+          const SlotList li = s_i[wave][ii];
+          const int oa = li.orig;
           // PD's arithmetic: dc = p_j - p_i, while-loop wrapping (neighbourlist.cpp:383-392)
           double ex = px[ob] - px[oa], ey = py[ob] - py[oa], ez = pz[ob] - pz[oa];
           int pimg = 13;
+          bool in = true;
           if (A.periodic) {
             for (int it = 0; it < 8 && ex > A.half[0]; it++) { ex -= A.L[0]; pimg -= 9; }
             for (int it = 0; it < 8 && ex < -A.half[0]; it++) { ex += A.L[0]; pimg += 9; }
@@ -657,42 +684,68 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
                  (mz - (wrap_digit(lj.aux, 2) - wrap_digit(li.aux, 2))) == sz;
           }
           const double er2 = ex * ex + ey * ey + ez * ez;
-          in = in && er2 <= A.rlist2;
-          img = pimg;
+          if (in && er2 <= A.rlist2) lm |= 1u << ii;
         }
       }
-      if (in) {
-        lm |= 1u << ii;
-        imgs |= (unsigned long long)(img & 31) << (5 * ii);
-        const int lo_aux = oa < ob ? li.aux : lj.aux;
-        if (abs(oa - ob) < A.mid || ((lo_aux >> 24) & 1)) {
+    }
+    // bond-order codes: pairs within the sparse-partner reach or with sparse partners
+    unsigned need = 0;
+#pragma unroll
+    for (int ii = 0; ii < kClusterSize; ii++) {
+      const SlotList li = s_i[wave][ii];
+      const int lo_aux = li.orig < ob ? li.aux : lj.aux;
+      need |= (((lm >> ii) & 1u) && (abs(li.orig - ob) < A.mid || ((lo_aux >> 24) & 1))) ? 1u << ii : 0u;
+    }
+    if (__ballot(need != 0u)) {
+#pragma unroll 1
+      for (int ii = 0; ii < kClusterSize; ii++) {
+        if ((need >> ii) & 1u) {
+          const SlotList li = s_i[wave][ii];
+          const int oa = li.orig;
+          const int lo_aux = oa < ob ? li.aux : lj.aux;
           const int code = bond_code(oa, ob, (lo_aux >> 24) & 1, A.mid, band, sp_off, sp_part, sp_ord);
           codes |= (unsigned)code << (2 * ii);
         }
       }
     }
-    // one entry per (j atom, image); the i atoms of a lane usually share one image
+    // one entry per (j atom, image)
     const unsigned rank = (unsigned)min(k, kGroupRanks);
-    unsigned rem = lm;
-    while (__ballot(rem != 0)) {
-      const bool has = rem != 0;
-      unsigned m = 0;
-      int limg = 0;
-      if (has) {
-        const int l = __ffs(rem) - 1;
-        limg = (int)((imgs >> (5 * l)) & 31ull);
-#pragma unroll
-        for (int q = 0; q < kClusterSize; q++)
-          if (((rem >> q) & 1u) && (int)((imgs >> (5 * q)) & 31ull) == limg) m |= 1u << q;
-        rem &= ~m;
-      }
+    if (i_uniform) {
+      const bool has = lm != 0u;
       const unsigned long long bb = __ballot(has);
       const int pos = cnt + __popcll(bb & ((1ull << lane) - 1ull));
       if (has && pos < A.stride) {
-        const unsigned sm = spread8(m);
-        E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), m | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
+        const unsigned sm = spread8(lm);
+        const int limg = jimg - wimg[0];
+        E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
       }
       cnt += __popcll(bb);
+    } else {
+      unsigned rem = lm;
+      while (__ballot(rem != 0)) {
+        const bool has = rem != 0;
+        unsigned m = 0;
+        int limg = 0;
+        if (has) {
+          const int l = __ffs(rem) - 1;
+          int wl = wimg[0];
+#pragma unroll
+          for (int q = 1; q < kClusterSize; q++)
+            if (q == l) wl = wimg[q];
+          limg = jimg - wl;
+#pragma unroll
+          for (int q = 0; q < kClusterSize; q++)
+            if (((rem >> q) & 1u) && wimg[q] == wl) m |= 1u << q;
+          rem &= ~m;
+        }
+        const unsigned long long bb = __ballot(has);
+        const int pos = cnt + __popcll(bb & ((1ull << lane) - 1ull));
+        if (has && pos < A.stride) {
+          const unsigned sm = spread8(m);
+          E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), m | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
+        }
+        cnt += __popcll(bb);
+      }
     }
   }
   if (lane == 0) {
