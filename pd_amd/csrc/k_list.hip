@@ -17,6 +17,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 namespace pdgpu {
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
                                                       int *__restrict__ orig_slot, int *__restrict__ cl_col) {
   __shared__ double sz[kSortLds];
   __shared__ int sa[kSortLds];
-  __shared__ int by_rank[kSortLds];
+  __shared__ int by_rank[kSortLds];       // position in sa/sz of the atom of each rank
+  __shared__ unsigned char slj[kSortLds];  // the atom has an LJ epsilon
   const int c = blockIdx.x;
   if (c >= ncol) return;
   const int s0 = col_atom_start[c], cnt = col_count[c];
@@ -255,6 +257,7 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
       const int a = col_atoms[s0 + k];
       sa[k] = a;
       sz[k] = wz[a];
+      slj[k] = (lj_eps == nullptr || lj_eps[a] != 0.0f) ? 1 : 0;
     }
   __syncthreads();
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
         const double zb = sz[m];
         rank += (zb < za || (zb == za && sa[m] < a)) ? 1 : 0;
       }
-      by_rank[rank] = a;
+      by_rank[rank] = k;
     } else {
       for (int m = 0; m < cnt; m++) {
         const int b = col_atoms[s0 + m];
@@ -287,13 +290,17 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
   // the LJ arithmetic for slots that hold none.
   for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
     const int base = r & ~(kClusterSize - 1), end = min(base + kClusterSize, cnt);
-    const int a = by_rank[r];
-    const bool lj = lj_eps == nullptr || lj_eps[a] != 0.0f;
+    const int ka = by_rank[r];
+    const int a = sa[ka];
+    const bool lj = slj[ka] != 0;
     int pos = 0, nlj = 0;
-    for (int q = base; q < end; q++) {
-      const bool lq = lj_eps == nullptr || lj_eps[by_rank[q]] != 0.0f;
-      nlj += lq ? 1 : 0;
-      if (q < r && lq == lj) pos++;
+#pragma unroll
+    for (int q = base; q < base + kClusterSize; q++) {
+      if (q < end) {
+        const bool lq = slj[by_rank[q]] != 0;
+        nlj += lq ? 1 : 0;
+        if (q < r && lq == lj) pos++;
+      }
     }
     if (!lj) pos += nlj;
     const int slot = col_cl_start[c] * kClusterSize + base + pos;
@@ -617,10 +624,21 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
   // image and each lane writes at most one entry per round.
   int wimg[kClusterSize];
   bool i_uniform = true;
+  // a pair can carry a bond-order code only if its atom indices are less than
+  // mid apart or one of the two has sparse partners: the cluster's index span
+  // widened by mid bounds the j atoms that need the lookup
+  int i_lo = INT_MAX, i_hi = INT_MIN;
+  bool i_sparse = false;
 #pragma unroll
   for (int ii = 0; ii < kClusterSize; ii++) {
-    wimg[ii] = wrap_image(s_i[wave][ii].aux);
-    if (s_i[wave][ii].orig >= 0 && wimg[ii] != wrap_image(s_i[wave][0].aux)) i_uniform = false;
+    const SlotList li = s_i[wave][ii];
+    wimg[ii] = wrap_image(li.aux);
+    if (li.orig >= 0) {
+      if (wimg[ii] != wrap_image(s_i[wave][0].aux)) i_uniform = false;
+      i_lo = min(i_lo, li.orig);
+      i_hi = max(i_hi, li.orig);
+      i_sparse = i_sparse || ((li.aux >> 24) & 1);
+    }
   }
   int cnt = 0;
   // the first round's j slot; each round loads the next round's while it tests
@@ -688,13 +706,16 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
         }
       }
     }
-    // bond-order codes: pairs within the sparse-partner reach or with sparse partners
+    // bond-order codes: pairs less than mid apart in index or with sparse partners
+    const bool maybe = lm != 0u && ((ob > i_lo - A.mid && ob < i_hi + A.mid) || ((lj.aux >> 24) & 1) || i_sparse);
     unsigned need = 0;
+    if (__ballot(maybe)) {
 #pragma unroll
-    for (int ii = 0; ii < kClusterSize; ii++) {
-      const SlotList li = s_i[wave][ii];
-      const int lo_aux = li.orig < ob ? li.aux : lj.aux;
-      need |= (((lm >> ii) & 1u) && (abs(li.orig - ob) < A.mid || ((lo_aux >> 24) & 1))) ? 1u << ii : 0u;
+      for (int ii = 0; ii < kClusterSize; ii++) {
+        const SlotList li = s_i[wave][ii];
+        const int lo_aux = li.orig < ob ? li.aux : lj.aux;
+        need |= (((lm >> ii) & 1u) && (abs(li.orig - ob) < A.mid || ((lo_aux >> 24) & 1))) ? 1u << ii : 0u;
+      }
     }
     if (__ballot(need != 0u)) {
 #pragma unroll 1
