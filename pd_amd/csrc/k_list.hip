@@ -252,6 +252,11 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
   if (c >= ncol) return;
   const int s0 = col_atom_start[c], cnt = col_count[c];
   const bool in_lds = cnt <= kSortLds;
+  // the unfilled slots of the column's last cluster hold no atom
+  if ((int)threadIdx.x < kClusterSize && (cnt & (kClusterSize - 1)) != 0) {
+    const int r = (cnt & ~(kClusterSize - 1)) + (int)threadIdx.x;
+    if (r >= cnt) slot_orig[col_cl_start[c] * kClusterSize + r] = -1;
+  }
   if (in_lds)
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
       const int a = col_atoms[s0 + k];
@@ -315,26 +320,6 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int count) {
   if (i < count) p[i] = v;
 }
 
-__global__ void k_cluster_bbox(const int *__restrict__ n_clusters, int max_clusters,
-                               const int *__restrict__ slot_orig, const double *__restrict__ wx,
-                               const double *__restrict__ wy, const double *__restrict__ wz,
-                               double *__restrict__ lo, double *__restrict__ hi) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= max_clusters || c >= *n_clusters) return;
-  double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-  for (int k = 0; k < kClusterSize; k++) {
-    int a = slot_orig[c * kClusterSize + k];
-    if (a < 0) continue;
-    l[0] = fmin(l[0], wx[a]); h[0] = fmax(h[0], wx[a]);
-    l[1] = fmin(l[1], wy[a]); h[1] = fmax(h[1], wy[a]);
-    l[2] = fmin(l[2], wz[a]); h[2] = fmax(h[2], wz[a]);
-  }
-  for (int d = 0; d < 3; d++) {
-    lo[3 * c + d] = l[d];
-    hi[3 * c + d] = h[d];
-  }
-}
-
 // Per-slot data for the pair search: the binning's wrapped position, the atom
 // index, the wrap counts (one signed byte per axis: wrapped = PD position +
 // count * box) and whether the atom has sparse bond-order partners (bit 24).
@@ -343,13 +328,18 @@ struct SlotList {
   int orig, aux;
 };
 
-__global__ void k_slot_pack(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
-                            const double *__restrict__ wx, const double *__restrict__ wy,
-                            const double *__restrict__ wz, const int *__restrict__ wrap,
-                            const int *__restrict__ sp_off, SlotList *__restrict__ sl) {
+// One thread per slot; the 8 lanes of a cluster also reduce its bounding box
+// (min/max of the wrapped positions of its atoms) with lane exchanges.
+__global__ __launch_bounds__(256) void k_slot_pack(int nslots, const int *__restrict__ n_clusters,
+                                                   const int *__restrict__ slot_orig, const double *__restrict__ wx,
+                                                   const double *__restrict__ wy, const double *__restrict__ wz,
+                                                   const int *__restrict__ wrap, const int *__restrict__ sp_off,
+                                                   SlotList *__restrict__ sl, double *__restrict__ lo,
+                                                   double *__restrict__ hi) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots || s >= *n_clusters * kClusterSize) return;
-  const int a = slot_orig[s];
+  // whole clusters of 8 lanes stay together: the slot count is a multiple of 8
+  const bool live = s < nslots && s < *n_clusters * kClusterSize;
+  const int a = live ? slot_orig[s] : -1;
   SlotList v;
   if (a >= 0) {
     v.wx = wx[a];
@@ -362,7 +352,23 @@ __global__ void k_slot_pack(int nslots, const int *__restrict__ n_clusters, cons
     v.orig = -1;
     v.aux = 0;
   }
-  sl[s] = v;
+  if (live) sl[s] = v;
+  double l[3] = {a >= 0 ? v.wx : 1e300, a >= 0 ? v.wy : 1e300, a >= 0 ? v.wz : 1e300};
+  double h[3] = {a >= 0 ? v.wx : -1e300, a >= 0 ? v.wy : -1e300, a >= 0 ? v.wz : -1e300};
+#pragma unroll
+  for (int off = 1; off < kClusterSize; off <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      l[d] = fmin(l[d], __shfl_xor(l[d], off, 64));
+      h[d] = fmax(h[d], __shfl_xor(h[d], off, 64));
+    }
+  if (live && (s & (kClusterSize - 1)) == 0) {
+    const int c = s / kClusterSize;
+    for (int d = 0; d < 3; d++) {
+      lo[3 * c + d] = l[d];
+      hi[3 * c + d] = h[d];
+    }
+  }
 }
 
 __device__ __forceinline__ int wrap_digit(int aux, int axis) { return (int)(int8_t)((aux >> (8 * axis)) & 0xFF); }
@@ -862,16 +868,13 @@ void Engine::build_list(bool deferred) {
   hipLaunchKernelGGL(k_fill_columns, dim3(nb_atoms), dim3(T), 0, stream, n, atom_col.p, col_atom_start.p,
                      col_fill.p, col_atoms.p);
   const int nslots = max_clusters * kClusterSize;
-  hipLaunchKernelGGL(k_fill_int, dim3((nslots + T - 1) / T), dim3(T), 0, stream, slot_orig.p, -1, nslots);
   hipLaunchKernelGGL(k_sort_columns, dim3(ncol), dim3(256), 0, stream, ncol, col_count.p, col_atom_start.p,
                      col_cl_start.p, col_atoms.p, wz.p, (have_nb && lj_e.n >= (size_t)n) ? lj_e.p : nullptr,
                      slot_orig.p, orig_slot.p, cl_col.p);
   slot_list.resize((size_t)nslots * sizeof(SlotList));
   hipLaunchKernelGGL(k_slot_pack, dim3((nslots + T - 1) / T), dim3(T), 0, stream, nslots, n_clusters_d.p,
                      slot_orig.p, wx.p, wy.p, wz.p, atom_wrap.p, sp_off.p,
-                     reinterpret_cast<SlotList *>(slot_list.p));
-  hipLaunchKernelGGL(k_cluster_bbox, dim3((max_clusters + T - 1) / T), dim3(T), 0, stream, n_clusters_d.p,
-                     max_clusters, slot_orig.p, wx.p, wy.p, wz.p, cl_lo.p, cl_hi.p);
+                     reinterpret_cast<SlotList *>(slot_list.p), cl_lo.p, cl_hi.p);
 
   // 3. pair search
   SearchArgs A{};

@@ -431,17 +431,40 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
         }
       }
     }
-    // i forces: the cluster is rank `wave` of its own group's table
+    // i forces: the cluster is rank `wave` of its own group's table.  The 24
+    // per-lane partials (component-major: q = 8 * component + atom) are summed
+    // over the wave by a reduce-scatter: each exchange step halves the values a
+    // lane holds and the partner lane's half is added, so lane groups of 8 end
+    // with 3 sums each, finished over the group (a fixed order throughout).
+    if (FORCES) {
+      float v[24];
 #pragma unroll
-    for (int a = 0; a < 8 && FORCES; a++) {
-      const float gx = wave_sum_f(fix[a]);
-      const float gy = wave_sum_f(fiy[a]);
-      const float gz = wave_sum_f(fiz[a]);
-      if (lane == a) {
-        const int l = wave * kClusterSize + a;
-        lds_fix_add(&jacc[0][l], fix_from_float(gx));
-        lds_fix_add(&jacc[1][l], fix_from_float(gy));
-        lds_fix_add(&jacc[2][l], fix_from_float(gz));
+      for (int a = 0; a < 8; a++) {
+        v[a] = fix[a];
+        v[8 + a] = fiy[a];
+        v[16 + a] = fiz[a];
+      }
+      const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+      float w12[12], w6[6], w3[3];
+#pragma unroll
+      for (int k = 0; k < 12; k++) w12[k] = (b5 ? v[12 + k] : v[k]) + __shfl_xor(b5 ? v[k] : v[12 + k], 32, 64);
+#pragma unroll
+      for (int k = 0; k < 6; k++) w6[k] = (b4 ? w12[6 + k] : w12[k]) + __shfl_xor(b4 ? w12[k] : w12[6 + k], 16, 64);
+#pragma unroll
+      for (int k = 0; k < 3; k++) w3[k] = (b3 ? w6[3 + k] : w6[k]) + __shfl_xor(b3 ? w6[k] : w6[3 + k], 8, 64);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        w3[k] += __shfl_xor(w3[k], 4, 64);
+        w3[k] += __shfl_xor(w3[k], 2, 64);
+        w3[k] += __shfl_xor(w3[k], 1, 64);
+      }
+      if ((lane & 7) == 0) {
+        const int q0 = (b5 ? 12 : 0) + (b4 ? 6 : 0) + (b3 ? 3 : 0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const int q = q0 + k;
+          lds_fix_add(&jacc[q >> 3][wave * kClusterSize + (q & 7)], fix_from_float(w3[k]));
+        }
       }
     }
   }
