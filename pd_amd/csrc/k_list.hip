@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
                                                       int *__restrict__ orig_slot, int *__restrict__ cl_col) {
   __shared__ double sz[kSortLds];
   __shared__ int sa[kSortLds];
-  __shared__ int by_rank[kSortLds];       // position in sa/sz of the atom of each rank
+  __shared__ int by_rank[kSortLds];       // position in sa/sz of the atom of each rank | LJ flag << 30
   __shared__ unsigned char slj[kSortLds];  // the atom has an LJ epsilon
   const int c = blockIdx.x;
   if (c >= ncol) return;
@@ -257,29 +257,46 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
     const int r = (cnt & ~(kClusterSize - 1)) + (int)threadIdx.x;
     if (r >= cnt) slot_orig[col_cl_start[c] * kClusterSize + r] = -1;
   }
+  // sort key: z, with NaN as +inf so that the ranks stay a permutation; ties
+  // are broken by the atom index
+  auto key = [](double z) { return z == z ? z : (double)INFINITY; };
+  // the LDS arrays are padded to a multiple of 8 with entries that rank after
+  // every atom, so the rank loop reads 8 at a time without bounds tests
+  const int cnt8 = (cnt + 7) & ~7;
   if (in_lds)
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-      const int a = col_atoms[s0 + k];
-      sa[k] = a;
-      sz[k] = wz[a];
-      slj[k] = (lj_eps == nullptr || lj_eps[a] != 0.0f) ? 1 : 0;
+    for (int k = threadIdx.x; k < cnt8; k += blockDim.x) {
+      if (k < cnt) {
+        const int a = col_atoms[s0 + k];
+        sa[k] = a;
+        sz[k] = key(wz[a]);
+        slj[k] = (lj_eps == nullptr || lj_eps[a] != 0.0f) ? 1 : 0;
+      } else {
+        sa[k] = INT_MAX;
+        sz[k] = (double)INFINITY;
+      }
     }
   __syncthreads();
   for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
     const int a = in_lds ? sa[k] : col_atoms[s0 + k];
-    const double za = in_lds ? sz[k] : wz[a];
+    const double za = in_lds ? sz[k] : key(wz[a]);
     int rank = 0;
-    if (in_lds) {  // uniform: the common case runs from LDS, unrolled
-#pragma unroll 8
-      for (int m = 0; m < cnt; m++) {
-        const double zb = sz[m];
-        rank += (zb < za || (zb == za && sa[m] < a)) ? 1 : 0;
+    if (in_lds) {  // uniform: the common case runs from LDS, 8 entries per step, no branches
+      for (int m0 = 0; m0 < cnt8; m0 += 8) {
+        double zb[8];
+        int ab[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          zb[u] = sz[m0 + u];
+          ab[u] = sa[m0 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) rank += ((zb[u] < za) | ((zb[u] == za) & (ab[u] < a))) ? 1 : 0;
       }
-      by_rank[rank] = k;
+      by_rank[rank] = k | ((int)slj[k] << 30);
     } else {
       for (int m = 0; m < cnt; m++) {
         const int b = col_atoms[s0 + m];
-        const double zb = wz[b];
+        const double zb = key(wz[b]);
         rank += (zb < za || (zb == za && b < a)) ? 1 : 0;
       }
       const int slot = col_cl_start[c] * kClusterSize + rank;
@@ -295,17 +312,15 @@ __global__ __launch_bounds__(256) void k_sort_columns(int ncol, const int *__res
   // the LJ arithmetic for slots that hold none.
   for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
     const int base = r & ~(kClusterSize - 1), end = min(base + kClusterSize, cnt);
-    const int ka = by_rank[r];
-    const int a = sa[ka];
-    const bool lj = slj[ka] != 0;
+    const int br = by_rank[r];
+    const int a = sa[br & 0x3FFFFFFF];
+    const bool lj = (br >> 30) != 0;
     int pos = 0, nlj = 0;
 #pragma unroll
     for (int q = base; q < base + kClusterSize; q++) {
-      if (q < end) {
-        const bool lq = slj[by_rank[q]] != 0;
-        nlj += lq ? 1 : 0;
-        if (q < r && lq == lj) pos++;
-      }
+      const bool lq = q < end && (by_rank[q] >> 30) != 0;
+      nlj += lq ? 1 : 0;
+      pos += (q < r && lq == lj) ? 1 : 0;
     }
     if (!lj) pos += nlj;
     const int slot = col_cl_start[c] * kClusterSize + base + pos;
