@@ -7,7 +7,8 @@
 //   cluster order (rebuilt with the list): slots of 8 atoms per cluster, atoms
 //       of a cluster taken from one grid column sorted by z.  slot_orig[slot]
 //       maps back to the PD index (-1 for padding); spos (f64 x,y,z + pad, 32 B)
-//       holds the positions gathered each step, sprm (f32 scaled charge, LJ
+//       holds the positions gathered each step (spos32: the same in f32, 16 B,
+//       for the prune), sprm (f32 scaled charge, LJ
 //       radius, sqrt(eps), 16 B) the pair parameters; nbx,nby,nbz (int64) the nonbonded force
 //       accumulators in slot order.  Work groups of up to 4 clusters of one
 //       column share a candidate j-cluster table (wg_cand).
@@ -294,6 +295,7 @@ struct Engine {
   DBuf<int> cl_col;                        // column of each cluster
   DBuf<double> cl_lo, cl_hi;               // cluster bounding boxes (wrapped frame), 3 per cluster
   DBuf<SlotPos> spos;                      // gathered FP64 positions, slot order
+  DBuf<float4> spos32;                     // the same positions in FP32 (x, y, z, 0) for the prune
   DBuf<float4> sprm;                       // per slot: scaled charge, LJ radius, sqrt(eps), 0
   int max_groups = 0;                      // allocation bound of work groups
   DBuf<int> col_grp_start, grp_first, grp_count, n_groups_d;

@@ -58,7 +58,8 @@ __device__ __forceinline__ double image_shift(int img, int axis, double L) {
 // for the next evaluation) and clears the force accumulators and energy slots.
 __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
                            const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                           SlotPos *__restrict__ spos, const SlotPos *__restrict__ pref, unsigned *__restrict__ dmax,
+                           SlotPos *__restrict__ spos, float4 *__restrict__ spos32, const SlotPos *__restrict__ pref,
+                           unsigned *__restrict__ dmax,
                            int cur, long long *__restrict__ nbx, long long *__restrict__ nby,
                            long long *__restrict__ nbz, long long *__restrict__ fbx, long long *__restrict__ fby,
                            long long *__restrict__ fbz, double *__restrict__ e_out, int n_e) {
@@ -93,6 +94,7 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
       }
       p.pad = 0.0;
       spos[s] = p;
+      spos32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
     }
   }
   d = wave_max(d);
@@ -116,11 +118,12 @@ constexpr int kPruneDepth = 4;
 __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
                                                float rk2, float margin_ok, int forced,
                                                const unsigned *__restrict__ dmax,  // this evaluation's words
-                                               const SlotPos *__restrict__ spos, const uint2 *__restrict__ entries,
+                                               const SlotPos *__restrict__ spos, const float4 *__restrict__ spos32,
+                                               const uint2 *__restrict__ entries,
                                                const int *__restrict__ cl_count, int stride,
                                                uint2 *__restrict__ pentries, int *__restrict__ pcount,
                                                SlotPos *__restrict__ pref) {
-  __shared__ double shtab[27 * 3];
+  __shared__ float shtab[27 * 3];
   __shared__ unsigned s_dmax;
   if (threadIdx.x < 64) {  // the largest of the spread words
     unsigned v = threadIdx.x < kDmaxSpread ? dmax[threadIdx.x * kDmaxStride] : 0u;
@@ -132,23 +135,23 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
   if (!forced && 2.0f * __uint_as_float(s_dmax) <= margin_ok) return;
   if (threadIdx.x < 81) {
     const int axis = threadIdx.x % 3;
-    shtab[threadIdx.x] = image_shift(threadIdx.x / 3, axis, axis == 0 ? Lx : (axis == 1 ? Ly : Lz));
+    shtab[threadIdx.x] = (float)image_shift(threadIdx.x / 3, axis, axis == 0 ? Lx : (axis == 1 ? Ly : Lz));
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (ci >= *n_clusters) return;  // whole waves; no barrier follows
   if (lane < kClusterSize) pref[ci * kClusterSize + lane] = spos[ci * kClusterSize + lane];
-  // FP32 offsets from the cluster's first atom: their rounding (~1e-6 A) is far
-  // inside the 1e-3 A allowance of the prune criterion
-  const SlotPos oi = spos[ci * kClusterSize];
+  // FP32 positions and offsets from the cluster's first atom: their rounding
+  // (~1e-5 A at 100 A) is far inside the 1e-3 A allowance of the prune criterion
+  const float4 oi = spos32[ci * kClusterSize];
   float iox[8], ioy[8], ioz[8];
 #pragma unroll
   for (int a = 0; a < 8; a++) {
-    const SlotPos p = spos[ci * kClusterSize + a];
-    iox[a] = (float)(p.x - oi.x);
-    ioy[a] = (float)(p.y - oi.y);
-    ioz[a] = (float)(p.z - oi.z);
+    const float4 p = spos32[ci * kClusterSize + a];
+    iox[a] = p.x - oi.x;
+    ioy[a] = p.y - oi.y;
+    ioz[a] = p.z - oi.z;
   }
   const int cnt = cl_count[ci];
   const uint2 *E = entries + (size_t)ci * stride;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
   int out = 0;
   for (int k0 = 0; k0 < cnt; k0 += 64 * kPruneDepth) {
     uint2 ent[kPruneDepth];
-    SlotPos pj[kPruneDepth];
+    float4 pj[kPruneDepth];
 #pragma unroll
     for (int u = 0; u < kPruneDepth; u++) {
       const int k = k0 + 64 * u + lane;
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
 #pragma unroll
     for (int u = 0; u < kPruneDepth; u++) {
       const int k = k0 + 64 * u + lane;
-      if (k < cnt) pj[u] = spos[ent[u].x & 0xFFFFFFu];
+      if (k < cnt) pj[u] = spos32[ent[u].x & 0xFFFFFFu];
     }
 #pragma unroll
     for (int u = 0; u < kPruneDepth; u++) {
@@ -173,9 +176,8 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
       unsigned keep = 0u;
       if (k < cnt) {
         const int img = (int)((ent[u].x >> 24) & 31u);
-        const float ax = (float)((pj[u].x - oi.x) + shtab[3 * img]),
-                    ay = (float)((pj[u].y - oi.y) + shtab[3 * img + 1]),
-                    az = (float)((pj[u].z - oi.z) + shtab[3 * img + 2]);
+        const float ax = (pj[u].x - oi.x) + shtab[3 * img], ay = (pj[u].y - oi.y) + shtab[3 * img + 1],
+                    az = (pj[u].z - oi.z) + shtab[3 * img + 2];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
           const float dx = ax - iox[a], dy = ay - ioy[a], dz = az - ioz[a];
@@ -607,13 +609,14 @@ void Engine::begin_evaluation(bool nonbonded) {
   e_out.resize(24);
   if (!nonbonded) {
     hipLaunchKernelGGL(k_begin_nb, dim3((std::max(n, 24) + T - 1) / T), dim3(T), 0, stream, 0, n, n_clusters_d.p,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-                       fbx.p, fby.p, fbz.p, e_out.p, 24);
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                       nullptr, fbx.p, fby.p, fbz.p, e_out.p, 24);
     return;
   }
   require(list_valid, "neighbour list not built");
   const int nslots = max_clusters * kClusterSize;
   spos.resize(nslots);
+  spos32.resize(nslots);
   pref.resize(nslots);
   const size_t ndm = 2 * kDmaxSpread * kDmaxStride;
   if (prune_dmax.n < ndm) {
@@ -623,7 +626,7 @@ void Engine::begin_evaluation(bool nonbonded) {
   dmax_cur ^= 1;
   const int m = std::max(std::max(nslots, n), 24);
   hipLaunchKernelGGL(k_begin_nb, dim3((m + T - 1) / T), dim3(T), 0, stream, nslots, n, n_clusters_d.p, slot_orig.p,
-                     x.p, y.p, z.p, spos.p, pref.p, prune_dmax.p, dmax_cur, nbx.p, nby.p, nbz.p, fbx.p, fby.p, fbz.p,
+                     x.p, y.p, z.p, spos.p, spos32.p, pref.p, prune_dmax.p, dmax_cur, nbx.p, nby.p, nbz.p, fbx.p, fby.p, fbz.p,
                      e_out.p, 24);
 }
 
@@ -650,7 +653,7 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     hipLaunchKernelGGL(k_prune, dim3((max_clusters + 3) / 4), dim3(256), 0, stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
                        (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur * kDmaxSpread * kDmaxStride, spos.p,
-                       entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
+                       spos32.p, entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
     prune_forced = false;
   }
   e_partial.resize((size_t)max_groups * 4);
