@@ -13,8 +13,11 @@
 //       accumulators in slot order.  Work groups of up to 4 clusters of one
 //       column share a candidate j-cluster table (wg_cand).
 //   list: for every i-cluster a contiguous run of 8-byte entries
-//       {j slot | image<<24 ; listed-mask(8) | bond-order codes(16)} (see
-//       k_list.hip), located by cl_entry_start / cl_entry_count.
+//       {j slot | image<<24 | j-has-LJ<<29 ; listed-mask(8) | bond-order codes(16)
+//       | candidate rank<<24} (see k_list.hip), located by cl_entry_start /
+//       cl_entry_count.  The pruned copy (pentries) keeps the entries whose j atom
+//       has an LJ epsilon from the front of each cluster's run (pcount of them)
+//       and the others from the back (pcount_nolj, last one first).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -307,7 +310,7 @@ struct Engine {
   // at the prune positions pref; valid while every atom stays within
   // prune_margin / 2 of pref (checked on the device each step)
   DBuf<uint2> pentries;
-  DBuf<int> pcount;
+  DBuf<int> pcount, pcount_nolj;
   DBuf<SlotPos> pref;
   DBuf<unsigned> prune_dmax;               // max displacement from pref (float bits): 2 alternating sets of spread words
   int dmax_cur = 0;                        // which prune_dmax word this evaluation fills

@@ -9,9 +9,11 @@
 // image loops that also produce the image number, r^2 = (x*x + y*y) + z*z with
 // no FMA, kept when r^2 <= (Cutoff+Padding)^2.  The result is the exact PD pair
 // set, stored per i-cluster as 8-byte entries:
-//   x: j slot (24 bits) | image (5 bits) << 24    image as seen from the i atom
+//   x: j slot (24 bits) | image (5 bits) << 24 | (j atom has an LJ epsilon) << 29
+//      (image as seen from the i atom)
 //   y: listed mask over the 8 i atoms (8 bits) | 2-bit bond-order code per i atom
-//      << 8 (0 = order 7, 1 = 1-2, 2 = 1-3, 3 = 1-4)
+//      << 8 (0 = order 7, 1 = 1-2, 2 = 1-3, 3 = 1-4) | rank of the j atom's
+//      cluster in the work group's candidate table << 24
 // Bond orders follow the list builder: the band matrix for |i-j| < MaxIndexDelta,
 // 7 otherwise, then the off-diagonal corrections (neighbourlist.cpp:179-220).
 #include "engine.hpp"
@@ -337,7 +339,8 @@ __global__ void k_fill_int(int *__restrict__ p, int v, int count) {
 
 // Per-slot data for the pair search: the binning's wrapped position, the atom
 // index, the wrap counts (one signed byte per axis: wrapped = PD position +
-// count * box) and whether the atom has sparse bond-order partners (bit 24).
+// count * box), whether the atom has sparse bond-order partners (bit 24) and
+// whether it has a Lennard-Jones epsilon (bit 25).
 struct SlotList {
   double wx, wy, wz;
   int orig, aux;
@@ -349,8 +352,8 @@ __global__ __launch_bounds__(256) void k_slot_pack(int nslots, const int *__rest
                                                    const int *__restrict__ slot_orig, const double *__restrict__ wx,
                                                    const double *__restrict__ wy, const double *__restrict__ wz,
                                                    const int *__restrict__ wrap, const int *__restrict__ sp_off,
-                                                   SlotList *__restrict__ sl, double *__restrict__ lo,
-                                                   double *__restrict__ hi) {
+                                                   const float *__restrict__ lj_eps, SlotList *__restrict__ sl,
+                                                   double *__restrict__ lo, double *__restrict__ hi) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   // whole clusters of 8 lanes stay together: the slot count is a multiple of 8
   const bool live = s < nslots && s < *n_clusters * kClusterSize;
@@ -361,7 +364,8 @@ __global__ __launch_bounds__(256) void k_slot_pack(int nslots, const int *__rest
     v.wy = wy[a];
     v.wz = wz[a];
     v.orig = a;
-    v.aux = wrap[a] | ((sp_off[a + 1] > sp_off[a]) ? (1 << 24) : 0);
+    v.aux = wrap[a] | ((sp_off[a + 1] > sp_off[a]) ? (1 << 24) : 0) |
+            ((lj_eps == nullptr || lj_eps[a] != 0.0f) ? (1 << 25) : 0);
   } else {
     v.wx = v.wy = v.wz = 0.0;
     v.orig = -1;
@@ -750,8 +754,9 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
         }
       }
     }
-    // one entry per (j atom, image)
+    // one entry per (j atom, image); bit 29 of the first word: the j atom has an LJ epsilon
     const unsigned rank = (unsigned)min(k, kGroupRanks);
+    const unsigned jlj = (unsigned)((lj.aux >> 25) & 1) << 29;
     if (i_uniform) {
       const bool has = lm != 0u;
       const unsigned long long bb = __ballot(has);
@@ -759,7 +764,7 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
       if (has && pos < A.stride) {
         const unsigned sm = spread8(lm);
         const int limg = jimg - wimg[0];
-        E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), lm | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
+        E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24) | jlj, lm | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
       }
       cnt += __popcll(bb);
     } else {
@@ -784,7 +789,7 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
         const int pos = cnt + __popcll(bb & ((1ull << lane) - 1ull));
         if (has && pos < A.stride) {
           const unsigned sm = spread8(m);
-          E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24), m | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
+          E[pos] = make_uint2((unsigned)sb | ((unsigned)limg << 24) | jlj, m | ((codes & (sm | (sm << 1))) << 8) | (rank << 24));
         }
         cnt += __popcll(bb);
       }
@@ -889,6 +894,7 @@ void Engine::build_list(bool deferred) {
   slot_list.resize((size_t)nslots * sizeof(SlotList));
   hipLaunchKernelGGL(k_slot_pack, dim3((nslots + T - 1) / T), dim3(T), 0, stream, nslots, n_clusters_d.p,
                      slot_orig.p, wx.p, wy.p, wz.p, atom_wrap.p, sp_off.p,
+                     (have_nb && lj_e.n >= (size_t)n) ? lj_e.p : nullptr,
                      reinterpret_cast<SlotList *>(slot_list.p), cl_lo.p, cl_hi.p);
 
   // 3. pair search
@@ -990,7 +996,7 @@ void Engine::export_pairs(std::vector<int> &counts, std::vector<uint32_t> &packe
     for (int k = 0; k < h_cnt[ci]; k++) {
       uint2 e = h_ent[(size_t)ci * entry_capacity + k];
       int sb = (int)(e.x & 0xFFFFFFu);
-      int img = (int)(e.x >> 24);
+      int img = (int)((e.x >> 24) & 31u);
       int ob = h_slot[sb];
       unsigned lm = e.y & 0xFFu, codes = e.y >> 8;
       for (int ii = 0; ii < kClusterSize; ii++) {

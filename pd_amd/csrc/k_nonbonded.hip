@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
                                                const uint2 *__restrict__ entries,
                                                const int *__restrict__ cl_count, int stride,
                                                uint2 *__restrict__ pentries, int *__restrict__ pcount,
-                                               SlotPos *__restrict__ pref) {
+                                               int *__restrict__ pcount_nolj, SlotPos *__restrict__ pref) {
   __shared__ float shtab[27 * 3];
   __shared__ unsigned s_dmax;
   if (threadIdx.x < 64) {  // the largest of the spread words
@@ -156,7 +156,9 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
   const int cnt = cl_count[ci];
   const uint2 *E = entries + (size_t)ci * stride;
   uint2 *PE = pentries + (size_t)ci * stride;
-  int out = 0;
+  // entries whose j atom has an LJ epsilon go to the front, the others to the
+  // back (last one first), so the nonbonded kernel runs whole rounds without LJ
+  int out = 0, out_nolj = 0;
   for (int k0 = 0; k0 < cnt; k0 += 64 * kPruneDepth) {
     uint2 ent[kPruneDepth];
     float4 pj[kPruneDepth];
@@ -185,12 +187,20 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
         }
         keep &= ent[u].y & 0xFFu;
       }
-      const unsigned long long b = __ballot(keep != 0u);
-      if (keep) PE[out + __popcll(b & ((1ull << lane) - 1ull))] = make_uint2(ent[u].x, (ent[u].y & ~0xFFu) | keep);
+      const bool jlj = (ent[u].x >> 29) & 1u;
+      const unsigned long long below = (1ull << lane) - 1ull;
+      const unsigned long long b = __ballot(keep != 0u && jlj), bn = __ballot(keep != 0u && !jlj);
+      const uint2 e = make_uint2(ent[u].x, (ent[u].y & ~0xFFu) | keep);
+      if (keep && jlj) PE[out + __popcll(b & below)] = e;
+      if (keep && !jlj) PE[stride - 1 - (out_nolj + __popcll(bn & below))] = e;
       out += __popcll(b);
+      out_nolj += __popcll(bn);
     }
   }
-  if (lane == 0) pcount[ci] = out;
+  if (lane == 0) {
+    pcount[ci] = out;
+    pcount_nolj[ci] = out_nolj;
+  }
 }
 
 __global__ void k_gather_params(int nslots, const int *__restrict__ n_clusters, const int *__restrict__ slot_orig,
@@ -212,7 +222,8 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
     const int *__restrict__ grp_count, const int *__restrict__ wg_ncand, const int *__restrict__ wg_cand,
     double Lx, double Ly, double Lz, const SlotPos *__restrict__ spos, const float4 *__restrict__ sprm,
     const uint2 *__restrict__ entries,
-    int stride, const int *__restrict__ cl_count, long long *__restrict__ fx, long long *__restrict__ fy,
+    int stride, const int *__restrict__ cl_count, const int *__restrict__ cl_count_nolj,
+    long long *__restrict__ fx, long long *__restrict__ fy,
     long long *__restrict__ fz, double *__restrict__ e_partial) {
   __shared__ long long jacc[3][kGroupRanks * kClusterSize];
   __shared__ int cand[kGroupRanks];
@@ -261,13 +272,18 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
     for (int a = 0; a < 8; a++) fix[a] = fiy[a] = fiz[a] = 0.0f;
     const float cut2f = (float)P.cut2, vcut2f = (float)P.vdwcut2;
 
-    const int cnt = cl_count[ci];
+    // the pruned entries: those whose j atom has an LJ epsilon from the front of
+    // the cluster's run, then the others from its back (last one first); rounds
+    // past the first group run without LJ arithmetic
+    const int cnt_lj = cl_count[ci];
+    const int cnt = cnt_lj + cl_count_nolj[ci];
     const uint2 *E = entries + (size_t)ci * stride;
-    uint2 nxt = lane < cnt ? E[lane] : make_uint2(0u, 0u);
+    auto at = [&](int k) { return k < cnt_lj ? k : stride - 1 - (k - cnt_lj); };
+    uint2 nxt = lane < cnt ? E[at(lane)] : make_uint2(0u, 0u);
     for (int k0 = 0; k0 < cnt; k0 += 64) {
       const bool valid = k0 + lane < cnt;
       const uint2 ent = nxt;
-      nxt = (k0 + 64 + lane < cnt) ? E[k0 + 64 + lane] : make_uint2(0u, 0u);  // prefetch the next round
+      nxt = (k0 + 64 + lane < cnt) ? E[at(k0 + 64 + lane)] : make_uint2(0u, 0u);  // prefetch the next round
       const int j = (int)(ent.x & 0xFFFFFFu);
       const int img = (int)((ent.x >> 24) & 31u);
       const unsigned lm = valid ? (ent.y & 0xFFu) : 0u;
@@ -326,11 +342,15 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           }
         }
       }
-      // The pair loop in two versions: rounds whose entries carry no bond-order
-      // code (no 1-2, 1-3 or 1-4 pair; most rounds) skip decoding the codes.
-      auto pair_loop = [&](auto has_codes, auto lj_first4) {
+      // The pair loop in six versions, chosen per round by wave-uniform tests:
+      // rounds whose entries carry no bond-order code (no 1-2, 1-3 or 1-4 pair;
+      // most rounds) skip decoding the codes, and the LJ arithmetic runs for
+      // slots 0..NLJ-1 only: none when no j atom of the round has an epsilon,
+      // 4 when the cluster's slots 4-7 hold none, else 8.  A skipped LJ term
+      // would be an exact zero, so every version gives the same bits.
+      auto pair_loop = [&](auto has_codes, auto lj_slots) {
         constexpr bool HC = decltype(has_codes)::value;
-        constexpr bool L4 = decltype(lj_first4)::value;
+        constexpr int NLJ = decltype(lj_slots)::value;  // LJ arithmetic for slots 0..NLJ-1
   #pragma unroll
         for (int a = 0; a < 8; a++) {
           // the displacement again (cheaper than keeping 24 registers live)
@@ -347,7 +367,7 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
           // A pair that is off has eps = qq = 0, so every energy and force term below
           // comes out as (signed) zero without a select of its own.
           float dEdr = 0.0f, Ev = 0.0f, Ee = 0.0f;
-          if (VDW && (!L4 || a < 4)) {
+          if (VDW && a < NLJ) {
             const float eps_ij = (s14 ? P.vdw14 : 1.0f) * pa.z * qj.z;
             // computed for every pair, then selected: a select of a product that
             // reads pa would be turned into a branch around the LDS read
@@ -408,14 +428,24 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
         }
       };
       const bool anycodes = __ballot(codes != 0u) != 0ull;
-      if (anycodes && lj4)
-        pair_loop(std::true_type{}, std::true_type{});
-      else if (anycodes)
-        pair_loop(std::true_type{}, std::false_type{});
-      else if (lj4)
-        pair_loop(std::false_type{}, std::true_type{});
-      else
-        pair_loop(std::false_type{}, std::false_type{});
+      using L0 = std::integral_constant<int, 0>;
+      using L4 = std::integral_constant<int, 4>;
+      using L8 = std::integral_constant<int, 8>;
+      if (k0 >= cnt_lj) {  // no j atom of the round has an LJ epsilon
+        if (anycodes)
+          pair_loop(std::true_type{}, L0{});
+        else
+          pair_loop(std::false_type{}, L0{});
+      } else if (anycodes) {
+        if (lj4)
+          pair_loop(std::true_type{}, L4{});
+        else
+          pair_loop(std::true_type{}, L8{});
+      } else if (lj4) {
+        pair_loop(std::false_type{}, L4{});
+      } else {
+        pair_loop(std::false_type{}, L8{});
+      }
       ev += (double)ev8;
       ee += (double)ee8;
       vir += (double)vir8;
@@ -506,7 +536,7 @@ void launch(Engine &E) {
                      E.n_groups_d.p, E.grp_first.p, E.grp_count.p, E.wg_ncand.p, E.wg_cand.p,
                      E.periodic ? E.box[0] : 0.0, E.periodic ? E.box[1] : 0.0, E.periodic ? E.box[2] : 0.0,
                      E.spos.p, E.sprm.p, E.pentries.p, (int)E.entry_capacity,
-                     E.pcount.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
+                     E.pcount.p, E.pcount_nolj.p, E.nbx.p, E.nby.p, E.nbz.p, E.e_partial.p);
 }
 
 template <int ELEC>
@@ -649,11 +679,13 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     ScopedTimer t(*this, "prune");
     pentries.resize((size_t)max_clusters * entry_capacity);
     pcount.resize(max_clusters);
+    pcount_nolj.resize(max_clusters);
     const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
     hipLaunchKernelGGL(k_prune, dim3((max_clusters + 3) / 4), dim3(256), 0, stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
                        (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur * kDmaxSpread * kDmaxStride, spos.p,
-                       spos32.p, entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p, pref.p);
+                       spos32.p, entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p,
+                       pcount_nolj.p, pref.p);
     prune_forced = false;
   }
   e_partial.resize((size_t)max_groups * 4);
