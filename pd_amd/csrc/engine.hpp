@@ -132,6 +132,9 @@ struct NBKernelParams {
   float invw_el, invw2_el4;
   float sA, sB, sC, sD, sE, fcut, eshift;  // force-switch constants (nonbonded.cpp:620-636)
   float rc2, rin2;
+  float fs_c1, fs_c2, fs_c3, fs_c4;  // sA*sB, sA*sC, sA*sD, sA*sE: the outer energy in Horner form
+  float fs_k;                        // rc2 - 3 rin2
+  int same_cut;                      // Cutoff == VdwCutoff: one cutoff test per pair
   float vdw14, elec14;
   int vdw_on;
   int elec_mode;      // 0 none, 1 plain, 3 energy switch, 4 force switch

@@ -317,14 +317,28 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       float r2a[8];
       bool inca[8], inva[8];
       bool anynear = false;
+      if (P.same_cut) {
+        // one cutoff: outside the band r2 differs from it, so both tests agree
 #pragma unroll
-      for (int a = 0; a < 8; a++) {
-        const float4 io = (&ioff[0][0])[ib + a];
-        const float dx = ax - io.x, dy = ay - io.y, dz = az - io.z;
-        r2a[a] = dx * dx + dy * dy + dz * dz;
-        inca[a] = r2a[a] <= cut2f;
-        inva[a] = r2a[a] < vcut2f;
-        anynear |= fabsf(r2a[a] - cut2f) < kNearBand || fabsf(r2a[a] - vcut2f) < kNearBand;
+        for (int a = 0; a < 8; a++) {
+          const float4 io = (&ioff[0][0])[ib + a];
+          const float dx = ax - io.x, dy = ay - io.y, dz = az - io.z;
+          r2a[a] = dx * dx + dy * dy + dz * dz;
+          const float d = r2a[a] - cut2f;
+          inca[a] = d <= 0.0f;
+          inva[a] = inca[a];
+          anynear |= fabsf(d) < kNearBand;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const float4 io = (&ioff[0][0])[ib + a];
+          const float dx = ax - io.x, dy = ay - io.y, dz = az - io.z;
+          r2a[a] = dx * dx + dy * dy + dz * dz;
+          inca[a] = r2a[a] <= cut2f;
+          inva[a] = r2a[a] < vcut2f;
+          anynear |= fabsf(r2a[a] - cut2f) < kNearBand || fabsf(r2a[a] - vcut2f) < kNearBand;
+        }
       }
       if (__builtin_expect(__ballot(anynear && lm != 0u) != 0ull, 0)) {
         const SlotPos pj = spos[j];
@@ -403,14 +417,18 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
               Ee = e0 * S;
               dEdr += d0;
             } else {  // force switch (Steinbach & Brooks), nonbonded.cpp:798-819
+              // outer: E = -qq (sA r (sB/r^2 + sC + sD r^2 + sE r^4) - fcut)
+              //          = qq (fcut - c1/r - r (c2 + r^2 (c3 + c4 r^2))),
+              //        dE/dr = -qq sA (rc^2 - r^2)^2 (rc^2 - 3 rin^2 + 2 r^2) / r^2
+              // inner: E = qq (1/r - eshift), dE/dr = -qq / r^2
               const bool outer = r > P.inner;
               const float dc = P.rc2 - r2f;
-              const float d_out = -qq * P.sA * rinv2 * dc * dc * (P.rc2 - 3.0f * P.rin2 + 2.0f * r2f);
-              const float e_out = -qq * (P.sA * r * (P.sB * rinv2 + P.sC + P.sD * r2f + P.sE * r2f * r2f) - P.fcut);
-              const float d_in = -qq * rinv2;
-              const float e_in = qq * (rinv - P.eshift);
-              Ee = outer ? e_out : e_in;
-              dEdr += outer ? d_out : d_in;
+              const float poly = fmaf(r2f, fmaf(P.fs_c4, r2f, P.fs_c3), P.fs_c2);
+              const float e_out = fmaf(-P.fs_c1, rinv, fmaf(-r, poly, P.fcut));
+              const float d_out = -P.sA * rinv2 * (dc * dc) * fmaf(2.0f, r2f, P.fs_k);
+              const float e_in = rinv - P.eshift;
+              Ee = qq * (outer ? e_out : e_in);
+              dEdr = fmaf(qq, outer ? d_out : -rinv2, dEdr);
             }
           }
           if (FORCES) {
@@ -600,6 +618,12 @@ void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
   k.fcut = (float)fcut;
   k.eshift = (float)eshift;
   k.rc2 = (float)(C * C);
+  k.fs_c1 = (float)(sA * sB);
+  k.fs_c2 = (float)(sA * sC);
+  k.fs_c3 = (float)(sA * sD);
+  k.fs_c4 = (float)(sA * sE);
+  k.fs_k = (float)(C * C - 3.0 * I * I);
+  k.same_cut = (p.Cutoff == p.VdwCutoff) ? 1 : 0;
   k.rin2 = (float)(I * I);
   k.vdw14 = (float)p.Vdw14Scaling;
   k.elec14 = (float)p.Elec14Scaling;
