@@ -262,8 +262,10 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
     }
     // The column sort puts a cluster's atoms with an LJ epsilon first; a cluster
     // whose slots 4..7 hold none (water: its hydrogens) skips LJ for them.
-    const bool lj4 =
-        (__ballot(lane >= 4 && lane < kClusterSize && sprm[ci * kClusterSize + lane].z != 0.0f) == 0ull);
+    // slots 0..lj_need-1 hold every atom of the cluster with an LJ epsilon
+    const unsigned lj_mask =
+        (unsigned)__ballot(lane < kClusterSize && sprm[ci * kClusterSize + lane].z != 0.0f) & 0xFFu;
+    const int lj_need = 32 - __clz((int)lj_mask);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -359,9 +361,10 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       // The pair loop in six versions, chosen per round by wave-uniform tests:
       // rounds whose entries carry no bond-order code (no 1-2, 1-3 or 1-4 pair;
       // most rounds) skip decoding the codes, and the LJ arithmetic runs for
-      // slots 0..NLJ-1 only: none when no j atom of the round has an epsilon,
-      // 4 when the cluster's slots 4-7 hold none, else 8.  A skipped LJ term
-      // would be an exact zero, so every version gives the same bits.
+      // slots 0..NLJ-1 only: none when no j atom of the round (or no i atom)
+      // has an epsilon, else 2, 3, 4 or 8 to cover the cluster's last slot that
+      // holds one (the column sort puts them first).  A skipped LJ term would be
+      // an exact zero, so every version gives the same bits.
       auto pair_loop = [&](auto has_codes, auto lj_slots) {
         constexpr bool HC = decltype(has_codes)::value;
         constexpr int NLJ = decltype(lj_slots)::value;  // LJ arithmetic for slots 0..NLJ-1
@@ -388,10 +391,10 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
             const float eps = (on && inv) ? eps_ij : 0.0f;
             const float Rij = pa.y + qj.y;
             const float s2 = Rij * Rij * rinv2;
-            const float B = s2 * s2 * s2;
-            const float A = B * B;
-            float e0 = eps * (A - 2.0f * B);
-            float d0 = -12.0f * eps * rinv * (A - B);
+            const float B = s2 * s2 * s2;  // A = B^2
+            const float eB = eps * B;
+            float e0 = eB * (B - 2.0f);                       // eps (A - 2B)
+            float d0 = (-12.0f * rinv) * fmaf(eB, B, -eB);   // -12 eps (A - B) / r
             const float t = fmaxf(r - P.vdwinner, 0.0f);
             const float S0 = 1.0f - (P.invw_vdw * t) * (P.invw_vdw * t);
             const float dS = -P.invw2_vdw4 * t * S0;
@@ -447,23 +450,28 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
       };
       const bool anycodes = __ballot(codes != 0u) != 0ull;
       using L0 = std::integral_constant<int, 0>;
+      using L2 = std::integral_constant<int, 2>;
+      using L3 = std::integral_constant<int, 3>;
       using L4 = std::integral_constant<int, 4>;
       using L8 = std::integral_constant<int, 8>;
-      if (k0 >= cnt_lj) {  // no j atom of the round has an LJ epsilon
-        if (anycodes)
-          pair_loop(std::true_type{}, L0{});
+      // no j atom of the round, or no i atom of the cluster, has an LJ epsilon
+      const int nlj = (k0 >= cnt_lj) ? 0 : lj_need;
+      auto by_lj = [&](auto hc) {
+        if (nlj == 0)
+          pair_loop(hc, L0{});
+        else if (nlj <= 2)
+          pair_loop(hc, L2{});
+        else if (nlj == 3)
+          pair_loop(hc, L3{});
+        else if (nlj == 4)
+          pair_loop(hc, L4{});
         else
-          pair_loop(std::false_type{}, L0{});
-      } else if (anycodes) {
-        if (lj4)
-          pair_loop(std::true_type{}, L4{});
-        else
-          pair_loop(std::true_type{}, L8{});
-      } else if (lj4) {
-        pair_loop(std::false_type{}, L4{});
-      } else {
-        pair_loop(std::false_type{}, L8{});
-      }
+          pair_loop(hc, L8{});
+      };
+      if (anycodes)
+        by_lj(std::true_type{});
+      else
+        by_lj(std::false_type{});
       ev += (double)ev8;
       ee += (double)ee8;
       vir += (double)vir8;
