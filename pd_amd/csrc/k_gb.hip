@@ -51,6 +51,25 @@ __device__ __forceinline__ bool tile_pair_valid(int lane, int t, int ni, int nj,
   return v;
 }
 
+// A bound on every (i, j) distance of a tile pair: the distance between the two
+// tiles' first atoms plus each tile's largest distance from its first atom.  A
+// tile pair whose bound (plus 1e-3 A) lies inside every cutoff (and inside the
+// switching radii) skips the per-pair tests: they would all pass, and the
+// switch would be exactly 1 with zero slope.  Such tiles also take their pair
+// displacements as FP32 differences of FP32 offsets from the i tile's first atom
+// (p0), within ~1e-6 A of PD's FP64 displacement.  Lane 0 holds a real i and j atom.
+__device__ __forceinline__ double tile_bound(double xi, double yi, double zi, bool iv, double xj, double yj,
+                                             double zj, bool jv, double &x0, double &y0, double &z0) {
+  x0 = __shfl(xi, 0, 64);
+  y0 = __shfl(yi, 0, 64);
+  z0 = __shfl(zi, 0, 64);
+  const double xj0 = __shfl(xj, 0, 64), yj0 = __shfl(yj, 0, 64), zj0 = __shfl(zj, 0, 64);
+  auto sq = [](double v) { return v * v; };
+  const double di = wave_max(iv ? sqrt(sq(xi - x0) + sq(yi - y0) + sq(zi - z0)) : 0.0);
+  const double dj = wave_max(jv ? sqrt(sq(xj - xj0) + sq(yj - yj0) + sq(zj - zj0)) : 0.0);
+  return sqrt(sq(xj0 - x0) + sq(yj0 - y0) + sq(zj0 - z0)) + di + dj + 1e-3;
+}
+
 // ---- pass 1: Born radii -------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict__ work, const GBSys *__restrict__ sys,
                                                  const double *__restrict__ x, const double *__restrict__ y,
@@ -59,6 +78,7 @@ __global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict
                                                  const int *__restrict__ sp_off, const int *__restrict__ sp_part,
                                                  const int *__restrict__ sp_ord, long long *__restrict__ gpol) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sxf[kTile], syf[kTile], szf[kTile];
   __shared__ float sR[kTile], sV[kTile], acc[kTile];
   const GBWork w = work[blockIdx.x];
   const GBSys S = sys[w.sys];
@@ -66,38 +86,56 @@ __global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict
   const int end = S.first + S.count;
   const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
   const bool diag = w.i0 == w.j0;
-  {
-    const int j = w.j0 + lane;
-    const bool v = lane < nj;
-    sx[lane] = v ? x[j] : 0.0;
-    sy[lane] = v ? y[j] : 0.0;
-    sz[lane] = v ? z[j] : 0.0;
-    sR[lane] = v ? gbrad[j] : 0.0f;
-    sV[lane] = v ? Vj[j] : 0.0f;
-    acc[lane] = 0.0f;
-  }
   const int i = w.i0 + lane;
   const bool iv = lane < ni;
   const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
   const float Ri = iv ? gbrad[i] : 0.0f, Vi = iv ? Vj[i] : 0.0f;
+  double x0, y0, z0;
+  bool fast;
+  {
+    const int j = w.j0 + lane;
+    const bool v = lane < nj;
+    const double xj = v ? x[j] : 0.0, yj = v ? y[j] : 0.0, zj = v ? z[j] : 0.0;
+    const double bd = tile_bound(xi, yi, zi, iv, xj, yj, zj, v, x0, y0, z0);
+    fast = bd * bd < K.still2;
+    sx[lane] = xj;
+    sy[lane] = yj;
+    sz[lane] = zj;
+    sxf[lane] = (float)(xj - x0);
+    syf[lane] = (float)(yj - y0);
+    szf[lane] = (float)(zj - z0);
+    sR[lane] = v ? gbrad[j] : 0.0f;
+    sV[lane] = v ? Vj[j] : 0.0f;
+    acc[lane] = 0.0f;
+  }
+  const float xif = (float)(xi - x0), yif = (float)(yi - y0), zif = (float)(zi - z0);
   __syncthreads();
   float gi = 0.0f;
+  auto tile = [&](auto fast_c) {
+  constexpr bool FAST = decltype(fast_c)::value;
   for (int t = 0; t < kTile; t++) {
     const int jj = (lane + t) & (kTile - 1);
     bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
-    double r2;
-    {
+    float r2f;
+    if (FAST) {
+      const float dx = xif - sxf[jj], dy = yif - syf[jj], dz = zif - szf[jj];
+      r2f = dx * dx + dy * dy + dz * dz;
+    } else {
+      double r2;
+      {
 #pragma clang fp contract(off)
-      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
-      r2 = dx * dx + dy * dy + dz * dz;  // sqrdist, gbff.cpp:455
+        const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
+        r2 = dx * dx + dy * dy + dz * dz;  // sqrdist, gbff.cpp:455
+      }
+      on = on && r2 <= K.still2;
+      r2f = (float)r2;
     }
-    on = on && r2 <= K.still2;
     if (w.special && on) {
       const int bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off,
                                    sp_off + S.sp_base, sp_part, sp_ord);
       on = bo > 2;  // only non-bonded and 1-4 partners (gbff.cpp:460-461)
     }
-    const float r2f = on ? (float)r2 : 1.0f;
+    r2f = on ? r2f : 1.0f;
     const float Rs = Ri + sR[jj];
     const float rdivR = r2f * __builtin_amdgcn_rcpf(Rs * Rs);  // hardware reciprocal, 1 ulp
     float ccf = 1.0f;
@@ -109,6 +147,11 @@ __global__ __launch_bounds__(64) void k_gb_radii(GBK K, const GBWork *__restrict
     gi += on ? base * sV[jj] : 0.0f;
     acc[jj] += on ? base * Vi : 0.0f;
   }
+  };
+  if (fast)
+    tile(std::true_type{});
+  else
+    tile(std::false_type{});
   __syncthreads();
   if (iv) fix_add(&gpol[i], fix_from_float(gi));
   if (lane < nj) fix_add(&gpol[w.j0 + lane], fix_from_float(acc[lane]));
@@ -147,6 +190,7 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const int *__restrict__ sp_ord, long long *__restrict__ fx, long long *__restrict__ fy,
     long long *__restrict__ fz, long long *__restrict__ deda, double *__restrict__ e_part) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sxf[kTile], syf[kTile], szf[kTile];
   __shared__ float sq[kTile], sr[kTile], se[kTile], sa[kTile];
   __shared__ float ax[kTile], ay[kTile], az[kTile], ad[kTile];
   const GBWork w = work[blockIdx.x];
@@ -155,49 +199,75 @@ __global__ __launch_bounds__(64) void k_gb_pair(
   const int end = S.first + S.count;
   const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
   const bool diag = w.i0 == w.j0;
+  const int i = w.i0 + lane;
+  const bool iv = lane < ni;
+  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
+  const float qi = iv ? q[i] : 0.0f, ri = iv ? ljr[i] : 0.0f, ei = iv ? lje[i] : 0.0f;
+  const float ai = iv ? born[i] : 1.0f;
+  double x0, y0, z0;
+  bool fast;
   {
     const int j = w.j0 + lane;
     const bool v = lane < nj;
-    sx[lane] = v ? x[j] : 0.0;
-    sy[lane] = v ? y[j] : 0.0;
-    sz[lane] = v ? z[j] : 0.0;
+    const double xj = v ? x[j] : 0.0, yj = v ? y[j] : 0.0, zj = v ? z[j] : 0.0;
+    // inside both cutoffs and both switching radii
+    const double bd = tile_bound(xi, yi, zi, iv, xj, yj, zj, v, x0, y0, z0);
+    fast = bd * bd < fmin(K.cut2, K.vdwcut2) && bd < (double)fminf(K.inner, K.vdwinner);
+    sx[lane] = xj;
+    sy[lane] = yj;
+    sz[lane] = zj;
+    sxf[lane] = (float)(xj - x0);
+    syf[lane] = (float)(yj - y0);
+    szf[lane] = (float)(zj - z0);
     sq[lane] = v ? q[j] : 0.0f;
     sr[lane] = v ? ljr[j] : 0.0f;
     se[lane] = v ? lje[j] : 0.0f;
     sa[lane] = v ? born[j] : 1.0f;
     ax[lane] = ay[lane] = az[lane] = ad[lane] = 0.0f;
   }
-  const int i = w.i0 + lane;
-  const bool iv = lane < ni;
-  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
-  const float qi = iv ? q[i] : 0.0f, ri = iv ? ljr[i] : 0.0f, ei = iv ? lje[i] : 0.0f;
-  const float ai = iv ? born[i] : 1.0f;
+  const float xif = (float)(xi - x0), yif = (float)(yi - y0), zif = (float)(zi - z0);
   __syncthreads();
   float fix = 0.0f, fiy = 0.0f, fiz = 0.0f, dai = 0.0f;
   double epol = 0.0, evdw = 0.0, eel = 0.0;
+  auto tile = [&](auto fast_c) {
+  constexpr bool FAST = decltype(fast_c)::value;
   for (int t = 0; t < kTile; t++) {
     const int jj = (lane + t) & (kTile - 1);
     bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
-    double dx, dy, dz, r2;
-    {
-#pragma clang fp contract(off)
-      dx = xi - sx[jj];
-      dy = yi - sy[jj];
-      dz = zi - sz[jj];
-      r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1588-1591
-    }
+    float dxf, dyf, dzf, r2raw;
+    double r2 = 0.0;
     bool on_gb = false;
-    if (EONLY) {
-      on_gb = on && sqrt(r2) <= K.cut;  // calcBornEnergy's test on the distance (gbff.cpp:560)
-      on = on && r2 <= K.cut2;
+    if (FAST) {
+      dxf = xif - sxf[jj];
+      dyf = yif - syf[jj];
+      dzf = zif - szf[jj];
+      r2raw = dxf * dxf + dyf * dyf + dzf * dzf;
+      if (EONLY) on_gb = on;
     } else {
-      on = on && r2 <= K.cut2;
+      double dx, dy, dz;
+      {
+#pragma clang fp contract(off)
+        dx = xi - sx[jj];
+        dy = yi - sy[jj];
+        dz = zi - sz[jj];
+        r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1588-1591
+      }
+      if (EONLY) {
+        on_gb = on && sqrt(r2) <= K.cut;  // calcBornEnergy's test on the distance (gbff.cpp:560)
+        on = on && r2 <= K.cut2;
+      } else {
+        on = on && r2 <= K.cut2;
+      }
+      dxf = (float)dx;
+      dyf = (float)dy;
+      dzf = (float)dz;
+      r2raw = (float)r2;
     }
     int bo = 7;
     if (w.special && on)
       bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off, sp_off + S.sp_base, sp_part,
                          sp_ord);
-    const float r2f = (on || on_gb) ? (float)r2 : 1.0f;
+    const float r2f = (on || on_gb) ? r2raw : 1.0f;
     const float rinv = __builtin_amdgcn_rsqf(r2f);
     const float r = r2f * rinv;
     const float aj = sa[jj];
@@ -206,11 +276,14 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const float uGB = -r2f * 0.25f * __builtin_amdgcn_rcpf(aiaj);
     const bool nb = bo >= 3;                      // vdW/elec only for non-bonded and 1-4 pairs
     const float s14e = bo == 3 ? K.elec14 : 1.0f;
-    // switching (r >= InnerCutoff), gbff.cpp:1640-1651
-    const float st = fmaxf(r - K.inner, 0.0f);
-    const float S0 = 1.0f - (K.invw * st) * (K.invw * st);
-    const float dS = -K.invw2_4 * st * S0;
-    const float Sw = S0 * S0;
+    // switching (r >= InnerCutoff), gbff.cpp:1640-1651; none inside a fast tile
+    float dS = 0.0f, Sw = 1.0f;
+    if (!FAST) {
+      const float st = fmaxf(r - K.inner, 0.0f);
+      const float S0 = 1.0f - (K.invw * st) * (K.invw * st);
+      dS = -K.invw2_4 * st * S0;
+      Sw = S0 * S0;
+    }
     float ep = 0.0f, dedr = 0.0f, dal = 0.0f, Eel = 0.0f, Fel = 0.0f;
     if (EONLY) {
       ep = on_gb ? K.premul * qq * __builtin_amdgcn_rsqf(r2f + aiaj * __expf(uGB)) * Sw : 0.0f;
@@ -238,7 +311,7 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       Eel *= Sw;
     }
     float Ev = 0.0f, Fv = 0.0f;
-    if (nb && r2 < K.vdwcut2) {
+    if (nb && (FAST || r2 < K.vdwcut2)) {
       const float eps = (bo == 3 ? K.vdw14 : 1.0f) * ei * se[jj];
       const float Rij = ri + sr[jj];
       const float s2 = Rij * Rij * rinv * rinv;
@@ -246,17 +319,19 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       const float A = B * B;
       Ev = eps * (A - 2.0f * B);
       Fv = -12.0f * eps * rinv * (A - B);
-      const float vt = fmaxf(r - K.vdwinner, 0.0f);
-      const float V0 = 1.0f - (K.vinvw * vt) * (K.vinvw * vt);
-      const float vdS = -K.vinvw2_4 * vt * V0;
-      const float VS = V0 * V0;
-      Fv = VS * Fv + Ev * vdS;
-      Ev *= VS;
+      if (!FAST) {
+        const float vt = fmaxf(r - K.vdwinner, 0.0f);
+        const float V0 = 1.0f - (K.vinvw * vt) * (K.vinvw * vt);
+        const float vdS = -K.vinvw2_4 * vt * V0;
+        const float VS = V0 * V0;
+        Fv = VS * Fv + Ev * vdS;
+        Ev *= VS;
+      }
     }
     if (!EONLY) {
       // gradient along d = p_i - p_j: GB part 2 dedr / r, vdW + elec (dE/dr) / r
       const float g = on ? (2.0f * dedr + (nb ? (Fv + Fel) : 0.0f)) * rinv : 0.0f;
-      const float gx = g * (float)dx, gy = g * (float)dy, gz = g * (float)dz;
+      const float gx = g * dxf, gy = g * dyf, gz = g * dzf;
       fix -= gx;
       fiy -= gy;
       fiz -= gz;
@@ -270,6 +345,11 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     evdw += on && nb ? (double)Ev : 0.0;
     eel += on && nb ? (double)Eel : 0.0;
   }
+  };
+  if (fast)
+    tile(std::true_type{});
+  else
+    tile(std::false_type{});
   __syncthreads();
   if (!EONLY && iv) {
     fix_add(&fx[i], fix_from_float(fix));
@@ -313,6 +393,7 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
                                                  long long *__restrict__ fx, long long *__restrict__ fy,
                                                  long long *__restrict__ fz) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ float sxf[kTile], syf[kTile], szf[kTile];
   __shared__ float sR[kTile], sV[kTile], sW[kTile];
   __shared__ float ax[kTile], ay[kTile], az[kTile];
   const GBWork w = work[blockIdx.x];
@@ -321,41 +402,64 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
   const int end = S.first + S.count;
   const int ni = min(kTile, end - w.i0), nj = min(kTile, end - w.j0);
   const bool diag = w.i0 == w.j0;
+  const int i = w.i0 + lane;
+  const bool iv = lane < ni;
+  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
+  const float Ri = iv ? gbrad[i] : 0.0f, Vi = iv ? Vj[i] : 0.0f, Wi = iv ? wgt[i] : 0.0f;
+  double x0, y0, z0;
+  bool fast;
   {
     const int j = w.j0 + lane;
     const bool v = lane < nj;
-    sx[lane] = v ? x[j] : 0.0;
-    sy[lane] = v ? y[j] : 0.0;
-    sz[lane] = v ? z[j] : 0.0;
+    const double xj = v ? x[j] : 0.0, yj = v ? y[j] : 0.0, zj = v ? z[j] : 0.0;
+    const double bd = tile_bound(xi, yi, zi, iv, xj, yj, zj, v, x0, y0, z0);
+    fast = bd * bd < K.still2;
+    sx[lane] = xj;
+    sy[lane] = yj;
+    sz[lane] = zj;
+    sxf[lane] = (float)(xj - x0);
+    syf[lane] = (float)(yj - y0);
+    szf[lane] = (float)(zj - z0);
     sR[lane] = v ? gbrad[j] : 0.0f;
     sV[lane] = v ? Vj[j] : 0.0f;
     sW[lane] = v ? wgt[j] : 0.0f;
     ax[lane] = ay[lane] = az[lane] = 0.0f;
   }
-  const int i = w.i0 + lane;
-  const bool iv = lane < ni;
-  const double xi = iv ? x[i] : 0.0, yi = iv ? y[i] : 0.0, zi = iv ? z[i] : 0.0;
-  const float Ri = iv ? gbrad[i] : 0.0f, Vi = iv ? Vj[i] : 0.0f, Wi = iv ? wgt[i] : 0.0f;
+  const float xif = (float)(xi - x0), yif = (float)(yi - y0), zif = (float)(zi - z0);
   __syncthreads();
   float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
+  auto tile = [&](auto fast_c) {
+  constexpr bool FAST = decltype(fast_c)::value;
   for (int t = 0; t < kTile; t++) {
     const int jj = (lane + t) & (kTile - 1);
     bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
-    double dx, dy, dz, r2;
-    {
+    float dxf, dyf, dzf, r2raw;
+    if (FAST) {
+      dxf = sxf[jj] - xif;
+      dyf = syf[jj] - yif;
+      dzf = szf[jj] - zif;
+      r2raw = dxf * dxf + dyf * dyf + dzf * dzf;
+    } else {
+      double dx, dy, dz, r2;
+      {
 #pragma clang fp contract(off)
-      dx = sx[jj] - xi;
-      dy = sy[jj] - yi;
-      dz = sz[jj] - zi;
-      r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1789-1792
+        dx = sx[jj] - xi;
+        dy = sy[jj] - yi;
+        dz = sz[jj] - zi;
+        r2 = dx * dx + dy * dy + dz * dz;  // gbff.cpp:1789-1792
+      }
+      on = on && r2 <= K.still2;
+      dxf = (float)dx;
+      dyf = (float)dy;
+      dzf = (float)dz;
+      r2raw = (float)r2;
     }
-    on = on && r2 <= K.still2;
     if (w.special && on) {
       const int bo = gb_bond_order(i - S.first, w.j0 + jj - S.first, S.mid, band + S.band_off,
                                    sp_off + S.sp_base, sp_part, sp_ord);
       on = bo > 2;
     }
-    const float r2f = on ? (float)r2 : 1.0f;
+    const float r2f = on ? r2raw : 1.0f;
     const float Rs = Ri + sR[jj];
     const float rdivR = r2f * __builtin_amdgcn_rcpf(Rs * Rs);
     float ccf = 1.0f, dccf = 0.0f;
@@ -370,7 +474,7 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
     // both directions of the pair: gpi deda_i P4 Vj + gpj deda_j P4 Vi
     const float r6 = r2f * r2f * r2f;
     const float dedr = on ? (Wi * sV[jj] + sW[jj] * Vi) * (4.0f * ccf - dccf) * __builtin_amdgcn_rcpf(r6) : 0.0f;
-    const float gx = dedr * (float)dx, gy = dedr * (float)dy, gz = dedr * (float)dz;
+    const float gx = dedr * dxf, gy = dedr * dyf, gz = dedr * dzf;
     fix -= gx;  // dedi_i += dedr (p_j - p_i); force = -dedi
     fiy -= gy;
     fiz -= gz;
@@ -378,6 +482,11 @@ __global__ __launch_bounds__(64) void k_gb_chain(GBK K, const GBWork *__restrict
     ay[jj] += gy;
     az[jj] += gz;
   }
+  };
+  if (fast)
+    tile(std::true_type{});
+  else
+    tile(std::false_type{});
   __syncthreads();
   if (iv) {
     fix_add(&fx[i], fix_from_float(fix));
