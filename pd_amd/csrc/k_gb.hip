@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     long long *__restrict__ fz, long long *__restrict__ deda, double *__restrict__ e_part) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
   __shared__ float sxf[kTile], syf[kTile], szf[kTile];
-  __shared__ float sq[kTile], sr[kTile], se[kTile], sa[kTile];
+  __shared__ float sq[kTile], sr[kTile], se[kTile], sa[kTile], sai[kTile];
   __shared__ float ax[kTile], ay[kTile], az[kTile], ad[kTile];
   const GBWork w = work[blockIdx.x];
   const GBSys S = sys[w.sys];
@@ -223,15 +223,21 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     sr[lane] = v ? ljr[j] : 0.0f;
     se[lane] = v ? lje[j] : 0.0f;
     sa[lane] = v ? born[j] : 1.0f;
+    sai[lane] = 1.0f / sa[lane];
     ax[lane] = ay[lane] = az[lane] = ad[lane] = 0.0f;
   }
   const float xif = (float)(xi - x0), yif = (float)(yi - y0), zif = (float)(zi - z0);
+  const float ai_q = 0.25f / ai;  // uGB = -r^2 / (4 a_i a_j) = -r^2 (0.25 / a_i) (1 / a_j)
   __syncthreads();
   float fix = 0.0f, fiy = 0.0f, fiz = 0.0f, dai = 0.0f;
   double epol = 0.0, evdw = 0.0, eel = 0.0;
   auto tile = [&](auto fast_c) {
   constexpr bool FAST = decltype(fast_c)::value;
-  for (int t = 0; t < kTile; t++) {
+  // energies: FP32 sums over 8 consecutive pairs of a lane, then FP64
+  for (int t0 = 0; t0 < kTile; t0 += 8) {
+  float ep8 = 0.0f, ev8 = 0.0f, el8 = 0.0f;
+#pragma unroll 1
+  for (int t = t0; t < t0 + 8; t++) {
     const int jj = (lane + t) & (kTile - 1);
     bool on = tile_pair_valid(lane, t, ni, nj, diag, jj);
     float dxf, dyf, dzf, r2raw;
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(64) void k_gb_pair(
     const float aj = sa[jj];
     const float qq = qi * sq[jj];
     const float aiaj = ai * aj;
-    const float uGB = -r2f * 0.25f * __builtin_amdgcn_rcpf(aiaj);
+    const float uGB = -r2f * (ai_q * sai[jj]);
     const bool nb = bo >= 3;                      // vdW/elec only for non-bonded and 1-4 pairs
     const float s14e = bo == 3 ? K.elec14 : 1.0f;
     // switching (r >= InnerCutoff), gbff.cpp:1640-1651; none inside a fast tile
@@ -341,9 +347,13 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       dai += on ? dal * aj : 0.0f;
       ad[jj] += on ? dal * ai : 0.0f;
     }
-    epol += (EONLY ? on_gb : on) ? (double)ep : 0.0;
-    evdw += on && nb ? (double)Ev : 0.0;
-    eel += on && nb ? (double)Eel : 0.0;
+    ep8 += (EONLY ? on_gb : on) ? ep : 0.0f;
+    ev8 += on && nb ? Ev : 0.0f;
+    el8 += on && nb ? Eel : 0.0f;
+  }
+  epol += (double)ep8;
+  evdw += (double)ev8;
+  eel += (double)el8;
   }
   };
   if (fast)
