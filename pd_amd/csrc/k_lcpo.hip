@@ -146,6 +146,7 @@ __global__ void k_lcpo_pack(int n, const double *__restrict__ x, const double *_
 // put those weights in LDS; the sums are reduced over the wave in a fixed
 // order.  EONLY: calcLCPOSasaEnergies (lcpo.cpp:522-556), the SASA alone, with
 // k excluded when (R_i + R_k) < d_ik.
+constexpr int kLcpoDepth = 4;  // (j, k) tasks per lane in flight in k_lcpo_sasa
 template <bool EONLY>
 __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double4 *__restrict__ pr, const int8_t *__restrict__ use,
                                                   const double4 *__restrict__ P,
@@ -218,39 +219,62 @@ __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double4 *__restri
   }
   __syncthreads();
   const int total = joff[ni];
+  // the (j, k) tasks, kLcpoDepth per lane at a time with every load of the
+  // batch issued before the arithmetic (the gathers, not the arithmetic, set the pace)
   int jn = 0;
-  for (int t = lane; t < total; t += 64) {
-    while (joff[jn + 1] <= t) jn++;  // t only grows, so j only moves forward
-    const int ja = jidx[jn];
-    const int kn = t - joff[jn];
-    const int ka = nl_idx[(size_t)ja * kLcpoMax + kn];
+  for (int t0 = lane; t0 < total; t0 += 64 * kLcpoDepth) {
+    int jnv[kLcpoDepth], kav[kLcpoDepth];
+    double sjkv[kLcpoDepth];
+#pragma unroll
+    for (int u = 0; u < kLcpoDepth; u++) {
+      const int t = t0 + 64 * u;
+      kav[u] = i;  // no task
+      jnv[u] = 0;
+      sjkv[u] = 0.0;
+      if (t < total) {
+        while (joff[jn + 1] <= t) jn++;  // t only grows, so j only moves forward
+        const int ja = jidx[jn];
+        const int kn = t - joff[jn];
+        jnv[u] = jn;
+        kav[u] = nl_idx[(size_t)ja * kLcpoMax + kn];
+        sjkv[u] = nl_s[(size_t)ja * kLcpoMax + kn];
+      }
+    }
+    double4 pkv[kLcpoDepth];
+#pragma unroll
+    for (int u = 0; u < kLcpoDepth; u++) pkv[u] = pr[kav[u]];
+#pragma unroll
+    for (int u = 0; u < kLcpoDepth; u++) {
+    const int ka = kav[u];
     if (ka == i) continue;
-    const double4 pk4 = pr[ka];
+    const int jn_u = jnv[u];
+    const double4 pk4 = pkv[u];
     const double pkx = pk4.x, pky = pk4.y, pkz = pk4.z;
     const double rk = pk4.w;
     const double hx = pkx - pix, hy = pky - piy, hz = pkz - piz;
     double dik = hx * hx + hy * hy + hz * hz;
-    const double sjk = nl_s[(size_t)ja * kLcpoMax + kn];
+    const double sjk = sjkv[u];
     if (EONLY) {
       if ((ri + rk) < sqrt(dik)) continue;
     } else {
       if ((ri + rk) * (ri + rk) < dik) continue;
     }
     Si3 += sjk;
-    Si4 += jw[jn][3] * sjk;
+    Si4 += jw[jn_u][3] * sjk;
     if (EONLY) continue;
     dik = sqrt(dik);
     const double invik = 1 / dik;
-    const double u = (ri * ri - rk * rk) * (invik * invik);
-    const double dA1 = PI * ri * (u - 1);
-    const double dA2 = PI * rk * (-u - 1);
-    const double s3ik = PI * ri * (2 * ri - dik - dik * u);
+    const double uu = (ri * ri - rk * rk) * (invik * invik);
+    const double dA1 = PI * ri * (uu - 1);
+    const double dA2 = PI * rk * (-uu - 1);
+    const double s3ik = PI * ri * (2 * ri - dik - dik * uu);
     const double dk[3] = {-(pix - pkx) * invik, -(piy - pky) * invik, -(piz - pkz) * invik};
-    const double c3 = jw[jn][1], c4 = jw[jn][2], sji = jw[jn][0];
+    const double c3 = jw[jn_u][1], c4 = jw[jn_u][2], sji = jw[jn_u][0];
     for (int c = 0; c < 3; c++) {
       der[c] += c3 * (dA1 * dk[c] + dA2 * dk[c]);                          // P3 term of SASA_j
-      der[c] += c4 * (sji * (dA1 * dk[c]) + sjk * dA2 * dk[c] + jw[jn][4 + c] * s3ik);  // P4 term of SASA_j
-      a4[c] += sjk * jw[jn][7 + c];
+      der[c] += c4 * (sji * (dA1 * dk[c]) + sjk * dA2 * dk[c] + jw[jn_u][4 + c] * s3ik);  // P4 term of SASA_j
+      a4[c] += sjk * jw[jn_u][7 + c];
+    }
     }
   }
   Si2 = wave_sum(Si2);
