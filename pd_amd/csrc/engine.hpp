@@ -178,6 +178,18 @@ struct StepRecord {
   double vdwcor, kcal_to_J;
 };
 
+constexpr int kMaxSums = 8;
+constexpr int kSumThreads = 256;
+struct SumList {
+  const double *part[kMaxSums];
+  double *out[kMaxSums];
+  int count[kMaxSums];
+  int n;
+  int record;
+  StepRecord rec;
+  const double *e_out;
+};
+
 // Device pointers of FF_Bonded terms (k_bonded.hip); force constants in kcal/mol units.
 struct BondedArrays {
   int nb, na, nt;
@@ -345,6 +357,11 @@ struct Engine {
   std::vector<PendingSum> pending_sums;
   void defer_sum4(const double *part, int count, double *out) { pending_sums.push_back({part, count, out}); }
   void flush_sums(const StepRecord *rec = nullptr);
+  // pdgpu_md: a step's sums (at most kMaxSums lists) wait for the next step's
+  // opening kernel, which runs them in its first workgroup
+  SumList fused_sums{};
+  bool fused_pending = false;
+  void defer_step_sums(const StepRecord &rec);
 
   // ---- bonded terms -------------------------------------------------------------
   int nbonds = 0, nangles = 0, ntors = 0;
@@ -492,6 +509,89 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Sums of 4-wide energy partials, all lists at once, by one workgroup; each
+// list is summed in a fixed order (thread-strided partial sums, then each wave,
+// then the wave sums in wave order). Then, for an MD step, the step's trace
+// entry: trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
+// per component slot; the kinetic energy slot is in J).
+// Run by one whole workgroup of kSumThreads threads (every thread calls it).
+__device__ __forceinline__ void sum_lists(const SumList &L) {
+  constexpr int T = kSumThreads;
+  constexpr int W = T / 64;
+  __shared__ double s[W][kMaxSums][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // each thread issues its first load of every list before any use, so the
+  // lists' global loads are in flight together; longer lists continue in a loop
+  double4 first[kMaxSums];
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    first[e] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (e < L.n && (int)threadIdx.x < L.count[e])
+      first[e] = reinterpret_cast<const double4 *>(L.part[e])[threadIdx.x];
+  }
+  double a[kMaxSums][4];
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    a[e][0] = first[e].x;
+    a[e][1] = first[e].y;
+    a[e][2] = first[e].z;
+    a[e][3] = first[e].w;
+    if (e < L.n) {
+      const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
+      int i = threadIdx.x + T;
+      for (; i + 3 * T < L.count[e]; i += 4 * T) {  // four loads in flight, added in order
+        double4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = P[i + u * T];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          a[e][0] += v[u].x;
+          a[e][1] += v[u].y;
+          a[e][2] += v[u].z;
+          a[e][3] += v[u].w;
+        }
+      }
+      for (; i < L.count[e]; i += T) {
+        const double4 v = P[i];
+        a[e][0] += v.x;
+        a[e][1] += v.y;
+        a[e][2] += v.z;
+        a[e][3] += v.w;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < kMaxSums; e++) {
+    if (e < L.n) {
+      for (int d = 0; d < 4; d++) {
+        const double t = wave_sum(a[e][d]);
+        if (lane == 0) s[w][e][d] = t;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 * kMaxSums) {
+    const int e = threadIdx.x >> 2, d = threadIdx.x & 3;
+    if (e < L.n) {
+      double t = 0.0;
+      for (int k = 0; k < W; k++) t += s[k][e][d];
+      L.out[e][d] = t;
+    }
+  }
+  __syncthreads();
+  if (L.record && threadIdx.x == 0) {
+    const double *E = L.e_out;
+    const StepRecord &R = L.rec;
+    double ep = 0.0;
+    if (R.use_sasa) ep += E[13];
+    if (R.use_nb) ep += E[0] + E[1] + R.vdwcor;
+    if (R.use_bonded) ep += E[4] + E[5] + E[6];
+    if (R.use_gb) ep += E[8] + E[9] + E[10] + E[12];
+    R.trace[2 * R.step] = ep * R.kcal_to_J;
+    R.trace[2 * R.step + 1] = E[20];
+  }
 }
 
 }  // namespace pdgpu

@@ -15,99 +15,9 @@ namespace pdgpu {
 
 namespace {
 
-// Sums of 4-wide energy partials, all lists at once, by one workgroup; each
-// list is summed in a fixed order (thread-strided partial sums, then each wave,
-// then the wave sums in wave order). Then, for an MD step, the step's trace
-// entry: trace[2*s] = epot (J), trace[2*s+1] = ekin (J) from e_out (kcal/mol, 4 doubles
-// per component slot; the kinetic energy slot is in J).
-constexpr int kMaxSums = 8;
-constexpr int kSumThreads = 256;
-struct SumList {
-  const double *part[kMaxSums];
-  double *out[kMaxSums];
-  int count[kMaxSums];
-  int n;
-  int record;
-  StepRecord rec;
-  const double *e_out;
-};
-
-template <int T>
-__global__ __launch_bounds__(T) void k_sums(SumList L) {
-  constexpr int W = T / 64;
-  __shared__ double s[W][kMaxSums][4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // each thread issues its first load of every list before any use, so the
-  // lists' global loads are in flight together; longer lists continue in a loop
-  double4 first[kMaxSums];
-#pragma unroll
-  for (int e = 0; e < kMaxSums; e++) {
-    first[e] = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (e < L.n && (int)threadIdx.x < L.count[e])
-      first[e] = reinterpret_cast<const double4 *>(L.part[e])[threadIdx.x];
-  }
-  double a[kMaxSums][4];
-#pragma unroll
-  for (int e = 0; e < kMaxSums; e++) {
-    a[e][0] = first[e].x;
-    a[e][1] = first[e].y;
-    a[e][2] = first[e].z;
-    a[e][3] = first[e].w;
-    if (e < L.n) {
-      const double4 *P = reinterpret_cast<const double4 *>(L.part[e]);
-      int i = threadIdx.x + T;
-      for (; i + 3 * T < L.count[e]; i += 4 * T) {  // four loads in flight, added in order
-        double4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = P[i + u * T];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          a[e][0] += v[u].x;
-          a[e][1] += v[u].y;
-          a[e][2] += v[u].z;
-          a[e][3] += v[u].w;
-        }
-      }
-      for (; i < L.count[e]; i += T) {
-        const double4 v = P[i];
-        a[e][0] += v.x;
-        a[e][1] += v.y;
-        a[e][2] += v.z;
-        a[e][3] += v.w;
-      }
-    }
-  }
-#pragma unroll
-  for (int e = 0; e < kMaxSums; e++) {
-    if (e < L.n) {
-      for (int d = 0; d < 4; d++) {
-        const double t = wave_sum(a[e][d]);
-        if (lane == 0) s[w][e][d] = t;
-      }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 4 * kMaxSums) {
-    const int e = threadIdx.x >> 2, d = threadIdx.x & 3;
-    if (e < L.n) {
-      double t = 0.0;
-      for (int k = 0; k < W; k++) t += s[k][e][d];
-      L.out[e][d] = t;
-    }
-  }
-  __syncthreads();
-  if (L.record && threadIdx.x == 0) {
-    const double *E = L.e_out;
-    const StepRecord &R = L.rec;
-    double ep = 0.0;
-    if (R.use_sasa) ep += E[13];
-    if (R.use_nb) ep += E[0] + E[1] + R.vdwcor;
-    if (R.use_bonded) ep += E[4] + E[5] + E[6];
-    if (R.use_gb) ep += E[8] + E[9] + E[10] + E[12];
-    R.trace[2 * R.step] = ep * R.kcal_to_J;
-    R.trace[2 * R.step + 1] = E[20];
-  }
-}
+// The energy sums of a step in their own launch (the last step of pdgpu_md, and
+// every evaluation outside it).
+__global__ __launch_bounds__(kSumThreads) void k_sums(SumList L) { sum_lists(L); }
 
 // Total force of atom i in J/m: the original-order accumulator plus the
 // nonbonded slot-order accumulator of the atom's slot.
@@ -593,10 +503,31 @@ void Engine::flush_sums(const StepRecord *rec) {
     L.record = (last && rec) ? 1 : 0;
     if (rec) L.rec = *rec;
     L.e_out = e_out.p;
-    hipLaunchKernelGGL(k_sums<kSumThreads>, dim3(1), dim3(kSumThreads), 0, stream, L);
+    hipLaunchKernelGGL(k_sums, dim3(1), dim3(kSumThreads), 0, stream, L);
   } while (k < pending_sums.size());
   pending_sums.clear();
   PDGPU_HIP(hipGetLastError());
+}
+
+// The sums of an MD step wait for the next step's opening kernel (k_begin_nb),
+// which saves a launch per step; more lists than one workgroup takes run now.
+void Engine::defer_step_sums(const StepRecord &rec) {
+  if (pending_sums.size() > (size_t)kMaxSums) {
+    flush_sums(&rec);
+    return;
+  }
+  SumList L{};
+  for (size_t k = 0; k < pending_sums.size(); k++, L.n++) {
+    L.part[L.n] = pending_sums[k].part;
+    L.out[L.n] = pending_sums[k].out;
+    L.count[L.n] = pending_sums[k].count;
+  }
+  L.record = 1;
+  L.rec = rec;
+  L.e_out = e_out.p;
+  pending_sums.clear();
+  fused_sums = L;
+  fused_pending = true;
 }
 
 // e_out layout (doubles): [0..3] nonbonded vdw, elec, virial, -; [4..7] bonded
@@ -606,6 +537,7 @@ void Engine::evaluate(uint32_t components, int step, bool flush) {
   const bool eonly = (components & PDGPU_ENERGY_ONLY) != 0;
   const bool nbon = (components & PDGPU_NONBONDED) != 0;
   pending_sums.clear();
+  if (flush) fused_pending = false;  // only pdgpu_md's steps hand their sums on
   if (nbon && !list_valid) build_list();
   begin_evaluation(nbon);
   if (nbon) launch_nonbonded(e_out.p + 0, !eonly);
@@ -816,6 +748,7 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
   require(p.timestep > 0, "timestep must be positive");
   require((p.components & PDGPU_ENERGY_ONLY) == 0, "MD needs forces: PDGPU_ENERGY_ONLY is for pdgpu_compute");
   trace.resize((size_t)std::max(n_steps, 1) * 2);
+  fused_pending = false;
   const uint32_t comp = p.components;
   const double cor = (comp & PDGPU_NONBONDED) ? nbk.vdwcor_per_atom * n : 0.0;
   for (int s = 0; s < n_steps; s++) {
@@ -833,7 +766,12 @@ void Engine::md(const pdgpu_md_params &p, int n_steps, double *epot_trace, doubl
       remove_momentum();
     const StepRecord rec{trace.p, s, (comp & PDGPU_NONBONDED) ? 1 : 0, (comp & PDGPU_BONDED) ? 1 : 0,
                          (comp & PDGPU_GB) ? 1 : 0, (comp & PDGPU_SASA) ? 1 : 0, cor, Consts::kcal_to_J()};
-    flush_sums(&rec);
+    defer_step_sums(rec);
+  }
+  if (fused_pending) {  // the last step's sums
+    hipLaunchKernelGGL(k_sums, dim3(1), dim3(kSumThreads), 0, stream, fused_sums);
+    PDGPU_HIP(hipGetLastError());
+    fused_pending = false;
   }
   std::vector<double> h((size_t)n_steps * 2);
   trace.download(h.data(), h.size(), stream);

@@ -62,10 +62,11 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
                            unsigned *__restrict__ dmax,
                            int cur, long long *__restrict__ nbx, long long *__restrict__ nby,
                            long long *__restrict__ nbz, long long *__restrict__ fbx, long long *__restrict__ fby,
-                           long long *__restrict__ fbz, double *__restrict__ e_out, int n_e) {
+                           long long *__restrict__ fbz, double *__restrict__ e_out, int n_e, int zero_e,
+                           SumList L) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (dmax && s < kDmaxSpread) dmax[((1 - cur) * kDmaxSpread + s) * kDmaxStride] = 0u;
-  if (s < n_e) e_out[s] = 0.0;
+  if (zero_e && s < n_e) e_out[s] = 0.0;
   if (s < n) {
     fbx[s] = 0;
     fby[s] = 0;
@@ -104,6 +105,9 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
   if ((threadIdx.x & 63) == 0 && d > 0.0)
     atomicMax(&dmax[(cur * kDmaxSpread + (blockIdx.x % kDmaxSpread)) * kDmaxStride],
               __float_as_uint((float)(d * (1.0 + 1e-6))));
+  // the previous MD step's energy sums and trace entry (their partials are not
+  // written again before this step's force kernels, which follow this one)
+  if (blockIdx.x == 0 && (L.n > 0 || L.record)) sum_lists(L);
 }
 
 // Rolling prune of the list: keeps the pairs within Cutoff + margin at the
@@ -669,10 +673,18 @@ void Engine::begin_evaluation(bool nonbonded) {
   fby.resize(n);
   fbz.resize(n);
   e_out.resize(24);
+  // a pending MD step's energy sums run in this kernel; e_out then keeps its
+  // values (every slot a component fills is assigned by the sums)
+  SumList L{};
+  const int zero_e = fused_pending ? 0 : 1;
+  if (fused_pending) {
+    L = fused_sums;
+    fused_pending = false;
+  }
   if (!nonbonded) {
     hipLaunchKernelGGL(k_begin_nb, dim3((std::max(n, 24) + T - 1) / T), dim3(T), 0, stream, 0, n, n_clusters_d.p,
                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
-                       nullptr, fbx.p, fby.p, fbz.p, e_out.p, 24);
+                       nullptr, fbx.p, fby.p, fbz.p, e_out.p, 24, zero_e, L);
     return;
   }
   require(list_valid, "neighbour list not built");
@@ -689,7 +701,7 @@ void Engine::begin_evaluation(bool nonbonded) {
   const int m = std::max(std::max(nslots, n), 24);
   hipLaunchKernelGGL(k_begin_nb, dim3((m + T - 1) / T), dim3(T), 0, stream, nslots, n, n_clusters_d.p, slot_orig.p,
                      x.p, y.p, z.p, spos.p, spos32.p, pref.p, prune_dmax.p, dmax_cur, nbx.p, nby.p, nbz.p, fbx.p, fby.p, fbz.p,
-                     e_out.p, 24);
+                     e_out.p, 24, zero_e, L);
 }
 
 // Per-slot parameters and accumulators; called once per list rebuild.
