@@ -400,6 +400,7 @@ struct SearchArgs {
   int periodic;
   int ncx, ncy;
   int Dx, Dy;                   // column search half-widths
+  double lo_x, lo_y, w_x, w_y;  // column grid (wrapped frame): column (cx, cy) covers lo + c w .. lo + (c+1) w
   double L[3];
   double half[3];
   double rlist2;                // (Cutoff+Padding)^2, PD's sqrIncludeLimit
@@ -501,8 +502,24 @@ __global__ __launch_bounds__(64 * kGroup) void k_pair_search(
         const int jcol = uy * A.ncx + ux;
         const int cs = col_cl_start[jcol], nc = col_ncl[jcol];
         const double shx = kx * A.L[0], shy = ky * A.L[1];
+        if (nc == 0) continue;
+        // the whole column (its grid rectangle, widened by 1e-3 A against rounding)
+        // out of reach of the group's box: no cluster of it can be a candidate
+        {
+          const double cx0 = A.lo_x + ux * A.w_x + shx - 1e-3, cx1 = cx0 + A.w_x + 2e-3;
+          const double cy0 = A.lo_y + uy * A.w_y + shy - 1e-3, cy1 = cy0 + A.w_y + 2e-3;
+          const double gx = fmax(0.0, fmax(cx0 - ghi[0], glo[0] - cx1));
+          const double gy = fmax(0.0, fmax(cy0 - ghi[1], glo[1] - cy1));
+          if (gx * gx + gy * gy > A.Rbox2) continue;
+        }
+        // the column's z extent: its clusters are z-ordered
+        const double czlo = cl_lo[3 * cs + 2], czhi = cl_hi[3 * (cs + nc - 1) + 2];
         for (int kz = kz_lo; kz <= kz_hi; kz++) {
           const double shz = kz * A.L[2];
+          {  // this image of the column out of reach in z
+            const double gz = fmax(0.0, fmax(czlo + shz - ghi[2], glo[2] - (czhi + shz)));
+            if (gz * gz > A.Rbox2) continue;
+          }
           const int sh = (kx + 1) * 9 + (ky + 1) * 3 + (kz + 1);
           for (int base = 0; base < nc; base += 64) {
             const int t = base + lane;
@@ -905,6 +922,10 @@ void Engine::build_list(bool deferred) {
   const double R = rlist + 1e-6 * std::max(1.0, rlist);
   A.Dx = (int)(R / g.w_x) + 1;
   A.Dy = (int)(R / g.w_y) + 1;
+  A.lo_x = g.lo_x;
+  A.lo_y = g.lo_y;
+  A.w_x = g.w_x;
+  A.w_y = g.w_y;
   // Unwrapped column offsets in [-D, D] visit each (column, image) at most once;
   // images beyond +-1 are skipped inside the kernel.
   for (int d = 0; d < 3; d++) {
