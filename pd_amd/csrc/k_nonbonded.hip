@@ -119,7 +119,8 @@ __global__ void k_begin_nb(int nslots, int n, const int *__restrict__ n_clusters
 // Four i-clusters per workgroup, one wave each; a wave takes 256 entries per
 // round (four per lane) with every load of the round issued before the tests.
 constexpr int kPruneDepth = 4;
-__global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
+constexpr int kPruneWaves = 4;  // i-clusters (one wave each) per workgroup
+__global__ __launch_bounds__(64 * kPruneWaves) void k_prune(const int *__restrict__ n_clusters, double Lx, double Ly, double Lz,
                                                float rk2, float margin_ok, int forced,
                                                const unsigned *__restrict__ dmax,  // this evaluation's words
                                                const SlotPos *__restrict__ spos, const float4 *__restrict__ spos32,
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_prune(const int *__restrict__ n_cluster
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ci = blockIdx.x * kPruneWaves + (threadIdx.x >> 6);
   if (ci >= *n_clusters) return;  // whole waves; no barrier follows
   if (lane < kClusterSize) pref[ci * kClusterSize + lane] = spos[ci * kClusterSize + lane];
   // FP32 positions and offsets from the cluster's first atom: their rounding
@@ -725,7 +726,8 @@ void Engine::launch_nonbonded(double *energy_out, bool forces) {
     pcount.resize(max_clusters);
     pcount_nolj.resize(max_clusters);
     const double rk = std::max(nb.Cutoff, nb.VdwCutoff) + prune_margin;
-    hipLaunchKernelGGL(k_prune, dim3((max_clusters + 3) / 4), dim3(256), 0, stream, n_clusters_d.p,
+    hipLaunchKernelGGL(k_prune, dim3((max_clusters + kPruneWaves - 1) / kPruneWaves), dim3(64 * kPruneWaves), 0,
+                       stream, n_clusters_d.p,
                        periodic ? box[0] : 0.0, periodic ? box[1] : 0.0, periodic ? box[2] : 0.0,
                        (float)(rk * rk), (float)(prune_margin - 1e-3), prune_forced ? 1 : 0, prune_dmax.p + dmax_cur * kDmaxSpread * kDmaxStride, spos.p,
                        spos32.p, entries.p, cl_entry_count.p, (int)entry_capacity, pentries.p, pcount.p,
