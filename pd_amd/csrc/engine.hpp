@@ -126,6 +126,7 @@ struct alignas(32) SlotPos {
 struct NBKernelParams {
   double cut2;        // Cutoff^2 (FP64: PD's per-pair test, nonbonded.cpp:720)
   double vdwcut2;     // VdwCutoff^2
+  double vdwcut_d;    // VdwCutoff: PD's LJ test is on the distance (nonbonded.cpp:742)
   float cut, inner, vdwcut, vdwinner;
   float invw2_vdw4;   // 4 / (VdwCutoff-VdwInnerCutoff)^2
   float invw_vdw;
@@ -148,6 +149,7 @@ struct GBK {
   double cut;           // Cutoff
   double still2;        // GbsaStillCutoff^2
   double vdwcut2;       // VdwCutoff^2
+  double vdwcut;        // VdwCutoff: PD's LJ test is on the distance (gbff.cpp:1686)
   float premul;         // -halfeconv (1/eps_in - 1/eps_out), kcal/mol A
   float ec_in, ec_out;  // econv / eps_in, econv / eps_out
   float inner, invw, invw2_4;

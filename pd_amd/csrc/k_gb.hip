@@ -317,7 +317,10 @@ __global__ __launch_bounds__(64) void k_gb_pair(
       Eel *= Sw;
     }
     float Ev = 0.0f, Fv = 0.0f;
-    if (nb && (FAST || r2 < K.vdwcut2)) {
+    // Dist_ij < VdwCutoff (gbff.cpp:1686): r^2 decides outside a 1e-12 relative
+    // band around VdwCutoff^2, the rounded square root inside it
+    if (nb && (FAST || r2 < K.vdwcut2 * (1.0 - 1e-12) ||
+               (r2 <= K.vdwcut2 * (1.0 + 1e-12) && sqrt(r2) < K.vdwcut))) {
       const float eps = (bo == 3 ? K.vdw14 : 1.0f) * ei * se[jj];
       const float Rij = ri + sr[jj];
       const float s2 = Rij * Rij * rinv * rinv;
@@ -603,6 +606,7 @@ void GBState::set_params(const pdgpu_gb_params &p) {
   k.cut = p.Cutoff;
   k.still2 = p.GbsaStillCutoff * p.GbsaStillCutoff;
   k.vdwcut2 = p.VdwCutoff * p.VdwCutoff;
+  k.vdwcut = p.VdwCutoff;
   k.premul = (float)(-Consts::halfeconv() * (1 / p.DielectricSolute - 1 / p.DielectricSolvent));
   k.ec_in = (float)(ec / p.DielectricSolute);
   k.ec_out = (float)(ec / p.DielectricSolvent);

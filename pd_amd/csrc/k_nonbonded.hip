@@ -358,8 +358,8 @@ __global__ __launch_bounds__(64 * kGroup) __attribute__((amdgpu_waves_per_eu(4, 
             const SlotPos pi = spos[ci * kClusterSize + a];
             const double ex = (pj.x - pi.x) + shx, ey = (pj.y - pi.y) + shy, ez = (pj.z - pi.z) + shz;
             const double r2 = ex * ex + ey * ey + ez * ez;
-            inca[a] = r2 <= P.cut2;
-            inva[a] = r2 < P.vdwcut2;
+            inca[a] = r2 <= P.cut2;              // sqrdistij > sqrcutoff (nonbonded.cpp:722)
+            inva[a] = sqrt(r2) < P.vdwcut_d;     // Dist_ij < VdwCutoff (nonbonded.cpp:742)
           }
         }
       }
@@ -600,6 +600,7 @@ void Engine::setup_nonbonded(const pdgpu_nb_params &p) {
   const double C = p.Cutoff, I = p.InnerCutoff;
   k.cut2 = C * C;
   k.vdwcut2 = p.VdwCutoff * p.VdwCutoff;
+  k.vdwcut_d = p.VdwCutoff;
   k.cut = (float)C;
   k.inner = (float)I;
   k.vdwcut = (float)p.VdwCutoff;

@@ -123,6 +123,28 @@ def test_water_forces(mode):
     assert abs(en["virial"] - e_ref["virial"]) <= 1e-5 * abs(e_ref["virial"])
 
 
+@pytest.mark.parametrize("cuts", [(10.0, 8.0, 8.0, 6.0), (8.0, 6.0, 10.0, 8.0)])
+def test_water_forces_distinct_cutoffs(cuts):
+    # VdwCutoff independent of Cutoff (nonbonded.cpp:722, 742): the kernel then
+    # tests both cutoffs per pair instead of one
+    c, ci, v, vi = cuts
+    kw = dict(WATER_NB, Cutoff=c, InnerCutoff=ci, VdwCutoff=v, VdwInnerCutoff=vi)
+    s = S.water_box(1000, seed=15)
+    print(f"case water1000 cutoffs={cuts}", flush=True)
+    pos, counts, packed, f_ref, e_ref = oracle_nb(s, kw, padding=2.0)
+    e = _engine(s, kw, 2.0)
+    e.build_list()
+    gc, gp = e.get_pairs()
+    assert np.array_equal(gc, counts) and np.array_equal(gp, packed)
+    f, en = e.compute(G.PDGPU_NONBONDED)
+    err = rel_rms(f, f_ref)
+    print(f"  rms force rel err {err:.2e}")
+    assert err <= 1e-5
+    for k in ("epot", "epot_vdw", "epot_elec"):
+        ref = e_ref[k] if k != "epot" else e_ref["epot_vdw"] + e_ref["epot_elec"]
+        assert energy_close(en[k], ref, e_ref["abs_sum"]), (k, en[k], ref, e_ref["abs_sum"])
+
+
 def test_peptide_vacuum_forces_with_14():
     s = S.peptide_chain(20, seed=3)
     kw = dict(Cutoff=12.0, InnerCutoff=10.0, VdwCutoff=12.0, VdwInnerCutoff=10.0, ForceSwitch=0,
