@@ -64,6 +64,9 @@ PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles
                            "water_box_98304_hbm_bytes.md")
 
 
+FLOPS_SUMMARY = os.path.join(os.path.dirname(PMC_SUMMARY), "valu_flops_counters.md")
+
+
 TIMING_EVERY = 7
 
 
@@ -78,6 +81,26 @@ def pmc_traffic(kernel):
                 if kernel in d:
                     return {"bytes_per_launch": d[kernel]["hbm_bytes_per_launch"],
                             "source": os.path.relpath(PMC_SUMMARY, os.path.dirname(PMC_SUMMARY) + "/..")}
+    except OSError:
+        pass
+    return {}
+
+
+def executed_flops(kernel, launch_s):
+    """Issued FP32 VALU rate of `kernel` over a measured launch time: the FLOP per
+    launch that the committed SQ-counter summary gives (64 lanes x (2 FMA + ADD +
+    MUL + TRANS) instructions, lanes masked off included), against the FP32 peak;
+    {} when the summary is absent."""
+    try:
+        for line in open(FLOPS_SUMMARY):
+            if line.startswith("FLOPS_JSON "):
+                d = json.loads(line[len("FLOPS_JSON "):])
+                if kernel in d and launch_s > 0:
+                    t = d[kernel]["fp32_flop_per_launch"] / launch_s / 1e12
+                    return {"tflops": round(t, 3), "frac": round(t / FP32_PEAK_TFLOPS, 5),
+                            "fp32_flop_per_launch": d[kernel]["fp32_flop_per_launch"],
+                            "valu_insts_per_launch": d[kernel]["valu_insts"],
+                            "source": os.path.relpath(FLOPS_SUMMARY, os.path.dirname(FLOPS_SUMMARY) + "/..")}
     except OSError:
         pass
     return {}
@@ -255,7 +278,8 @@ def gb_benchmarks(args, log):
         "kernel_us": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in kt.items()},
         "roofline_gb_pair": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                              "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                             "algorithmic_flop_per_launch": pairs * GB_FLOP_PAIR},
+                             "algorithmic_flop_per_launch": pairs * GB_FLOP_PAIR,
+                             "executed": executed_flops("k_gb_pair (5,111-atom chain)", pair_us * 1e-6)},
     }
     log(f"single chain: {out['single_chain']['steps_per_s']} steps/s ({time.perf_counter() - t0:.1f} s)")
     if not args.no_cpu_baseline:
@@ -463,6 +487,7 @@ def main():
             "launches_timed": nb_launch,
             "timed_every": TIMING_EVERY,
             "algorithmic_flop_per_launch": int(flops),
+            "executed": executed_flops("k_nonbonded (water)", nb_avg_s),
             "list_entries": int(n_tot),
             "pairs_in_cutoff": int(n_cut),
         },
