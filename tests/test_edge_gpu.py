@@ -1,0 +1,73 @@
+"""Edge cases of the nonbonded path against the FP64 oracle, at the same bars as
+the water tests (pair set and cleanSpace positions bit-exact, RMS force error
+at most 1e-5 of the RMS force, energies within 1e-5 relative plus the FP32
+summation allowance):
+
+- molecules stored several boxes away from the primary box: PD keeps positions
+  unwrapped and moves molecule centres into the box in cleanSpace
+  (space.cpp, PeriodicBox::cleanSpace), then takes minimum images;
+- a box holding two waters: fewer atoms than one 8-slot cluster, one grid
+  column, every neighbour column an image of the same one;
+- atoms without an LJ epsilon, and atoms without a charge.
+"""
+import numpy as np
+import pytest
+
+from pd_amd import engine as G
+from pd_amd import systems as S
+from tests.helpers import energy_close, oracle_nb, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+WATER_NB = dict(Cutoff=10.0, InnerCutoff=8.0, VdwCutoff=10.0, VdwInnerCutoff=8.0, ForceSwitch=1)
+
+
+def _check(s, kw, padding=2.0):
+    pos, counts, packed, f_ref, e_ref = oracle_nb(s, kw, padding=padding)
+    e = G.engine_for(s, G.nb_params(**kw), padding=padding)
+    e.build_list()
+    assert np.array_equal(e.get_positions(), pos), "cleanSpace positions differ"
+    gc, gp = e.get_pairs()
+    assert np.array_equal(gc, counts) and np.array_equal(gp, packed)
+    f, en = e.compute(G.PDGPU_NONBONDED)
+    err = rel_rms(f, f_ref)
+    print(f"  rms force rel err {err:.2e}; epot {en['epot']:.6e}", flush=True)
+    assert err <= 1e-5
+    for k in ("epot", "epot_vdw", "epot_elec"):
+        ref = e_ref[k] if k != "epot" else e_ref["epot_vdw"] + e_ref["epot_elec"]
+        assert energy_close(en[k], ref, e_ref["abs_sum"]), (k, en[k], ref)
+    return f, en, f_ref, e_ref
+
+
+def test_molecules_outside_the_box():
+    s = S.water_box(1000, seed=31)
+    rng = np.random.default_rng(31)
+    shifts = rng.integers(-3, 4, size=(s.n // 3, 3)) * s.box
+    s.pos = s.pos + np.repeat(shifts, 3, axis=0)
+    print("case water1000 shifted by -3..3 boxes per molecule", flush=True)
+    _check(s, WATER_NB)
+
+
+def test_two_waters_in_one_column():
+    s = S.water_box(2, seed=4, box_len=25.0)
+    # the second water 3.1 A from the first along x, across the box face
+    d = np.array([3.1 - s.box[0], 0.4, -0.2]) + s.pos[0] - s.pos[3]
+    s.pos[3:6] += d
+    print("case two waters, 25 A box", flush=True)
+    f, en, f_ref, e_ref = _check(s, WATER_NB)
+    assert abs(e_ref["epot_elec"]) > 0.0 and abs(e_ref["epot_vdw"]) > 0.0
+    # Newton's third law between the two molecules
+    assert np.abs(f[:3].sum(axis=0) + f[3:].sum(axis=0)).max() <= 1e-5 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("drop", ["epsilon", "charge"])
+def test_water_without_lj_or_charges(drop):
+    s = S.water_box(512, seed=41)
+    if drop == "epsilon":  # every cluster and every entry takes the LJ-free loop
+        s.epsilon = np.zeros_like(s.epsilon)
+    else:
+        s.charge = np.zeros_like(s.charge)
+    print(f"case water512 without {drop}", flush=True)
+    f, en, f_ref, e_ref = _check(s, WATER_NB)
+    zero = "epot_vdw" if drop == "epsilon" else "epot_elec"
+    assert en[zero] == 0.0 and e_ref[zero] == 0.0
