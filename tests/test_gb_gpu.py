@@ -38,9 +38,12 @@ def gpu_gb(s, kw):
     return e, f, en
 
 
+# 1 residue (22 atoms): one partly filled 64-atom tile; 4 residues (73 atoms):
+# a full tile and a partly filled one
 @pytest.mark.parametrize("n_res,seed,kw", [(20, 1, ALL_PAIRS), (20, 2, SWITCHED), (60, 3, ALL_PAIRS),
-                                           (60, 4, SWITCHED)], ids=["ala20-all", "pep20-switch",
-                                                                    "pep60-all", "pep60-switch"])
+                                           (60, 4, SWITCHED), (1, 5, ALL_PAIRS), (4, 6, SWITCHED)],
+                         ids=["ala20-all", "pep20-switch", "pep60-all", "pep60-switch", "pep1-all",
+                              "pep4-switch"])
 def test_gb_forces_energies(n_res, seed, kw):
     s = S.peptide_chain(n_res, seed=seed)
     print(f"case peptide{n_res} seed={seed} {kw['Cutoff']}", flush=True)
