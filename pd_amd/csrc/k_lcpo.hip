@@ -147,6 +147,10 @@ __global__ void k_lcpo_pack(int n, const double *__restrict__ x, const double *_
 // order.  EONLY: calcLCPOSasaEnergies (lcpo.cpp:522-556), the SASA alone, with
 // k excluded when (R_i + R_k) < d_ik.
 constexpr int kLcpoDepth = 4;  // (j, k) tasks per lane in flight in k_lcpo_sasa
+// waves of k_lcpo_sasa (each walks atoms blockIdx.x, + gridDim.x, ...): on the
+// 256-chain batch (285,363 atoms) 4096 / 8192 / 16384 / 65536 / one wave per atom
+// gave 197.6k / 199.7k / 202.9k / 204.6k / 199.6k aggregate steps/s
+constexpr int kLcpoGrid = 65536;
 template <bool EONLY>
 __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double4 *__restrict__ pr, const int8_t *__restrict__ use,
                                                   const double4 *__restrict__ P,
@@ -160,8 +164,8 @@ __global__ __launch_bounds__(64) void k_lcpo_sasa(int n, const double4 *__restri
   __shared__ double jw[kLcpoMax][10];  // s_ji, P3 sigma, P4 sigma, Si2_t, d2(3), a2t(3)
   const int lane = threadIdx.x;
   if (!lcpo_full(step_ptr, step_host, update)) return;  // uniform: the step is the same for all
-  // a grid of a few thousand waves walks the atoms, so the launches on the
-  // steps that re-apply stored forces cost almost nothing
+  // at most kLcpoGrid waves walk the atoms; on the steps that re-apply stored
+  // forces every wave returns at the test above
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
   __syncthreads();  // the LDS tables of the previous atom are no longer read
   if (!use[i]) {
@@ -392,14 +396,14 @@ void LCPOState::run(hipStream_t s, const double *x, const double *y, const doubl
   if (energy_only) {
     hipLaunchKernelGGL(k_lcpo_lists<true>, dim3(lb), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, pr.p,
                        step_ptr, step_host, upd, 2.0 * max_R, nl_cnt.p, nl_idx.p, nl_s.p);
-    hipLaunchKernelGGL(k_lcpo_sasa<true>, dim3(std::min(n, 8192)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
+    hipLaunchKernelGGL(k_lcpo_sasa<true>, dim3(std::min(n, kLcpoGrid)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
                        upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
     hipLaunchKernelGGL(k_lcpo_energy, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
                        sig.p, eatom.p);
   } else {
     hipLaunchKernelGGL(k_lcpo_lists<false>, dim3(lb), dim3(64), 0, s, n, atom_sys.p, sys_first.p, sys_count.p, pr.p,
                        step_ptr, step_host, upd, 2.0 * max_R, nl_cnt.p, nl_idx.p, nl_s.p);
-    hipLaunchKernelGGL(k_lcpo_sasa<false>, dim3(std::min(n, 8192)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
+    hipLaunchKernelGGL(k_lcpo_sasa<false>, dim3(std::min(n, kLcpoGrid)), dim3(64), 0, s, n, pr.p, use.p, P.p, sig.p, step_ptr, step_host,
                        upd, nl_cnt.p, nl_idx.p, nl_s.p, sasa.p, dvx.p, dvy.p, dvz.p);
     hipLaunchKernelGGL(k_lcpo_apply, dim3((n + 255) / 256), dim3(256), 0, s, n, Consts::J_to_kcalmol(), sasa.p,
                        sig.p, dvx.p, dvy.p, dvz.p, fx, fy, fz, eatom.p);
